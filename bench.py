@@ -1,0 +1,205 @@
+"""Benchmark: batched OS-EM slice-iterations/s on one MI355X.
+
+Workload: a batch of B slices of 1024 x 1024, 1200 projections over [0, 180),
+1024 bins, 10 ordered subsets (the 1024^2 / 1200-projection case of the
+batched OS-EM stack).  A step is one OS-EM iteration (all 10 subsets) over the
+whole batch through ctr_osem_batched, with the sinograms and images already
+resident in HBM; value = B * steps / time.
+
+Printed beside it:
+  per_slice_launch   the same kernels launched one slice at a time with a
+                     synchronisation after every kernel, as GPU.cpp does
+                     (finish() after each enqueue, GPU.cpp:216-227)
+  relerr_*           one slice-iteration of slice 0 against the CPU oracle's
+                     FP32 compat restatement and its FP64 version
+  roofline           the forward-projection kernel (the dominant one): its
+                     algorithmic FLOPs per launch over its mean launch time,
+                     measured with HIP events on the library stream
+  cpu_baseline       the oracle's FP32 restatement (OpenMP, all host threads)
+                     timed on one slice-iteration of slice 0
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ct-xafs_analysis_amd"))
+
+METRIC = "batched OS-EM slice-iters/s vs per-slice launch, 1024² 1200 proj, relerr≤1e-4"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# HBM bytes per forward-projection launch at this workload (128 x 1024^2 slices),
+# measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the same
+# counters on a plain copy kernel (profiles/fp_kernel_1024_counters.md).
+FP_TRAFFIC_BYTES_PER_SLICE = None
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_batch(B, n, pa, seed, device):
+    import numpy as np
+    import torch
+
+    from ctrecon import phantom as ph
+    ang = ph.uniform_angles(pa)
+    ells = [ph.random_ellipses(10, np.random.default_rng(ph.stream_seed(seed, i))) for i in range(B)]
+    sinos = []
+    chunk = 16
+    for c0 in range(0, B, chunk):
+        p = ph.line_integrals_torch(ells[c0:c0 + chunk], n, ang, n, device=device)
+        y = ph.transmission_noise_torch(p, 1.0e4, ph.stream_seed(seed, c0, 7))
+        sinos.extend(y.float().contiguous().unbind(0))
+        del p, y
+    sinos = [s.contiguous() for s in sinos]
+    return ang, sinos
+
+
+def cpu_baseline(n, pa, ss, ang, sino0, gpu_slice0):
+    """FP32 compat oracle and FP64 oracle on one slice-iteration of slice 0."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    g = O.geom(n, n, pa, ss=ss)
+    x0 = np.ones((1, n, n), np.float32)
+    t0 = time.perf_counter()
+    c32 = O.osem_execution(g, ang, x0, sino0[None], 1)
+    t32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c64 = O.osem_execution(g, ang, x0, sino0[None], 1, dtype=np.float64)
+    t64 = time.perf_counter() - t0
+    rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))  # noqa: E731
+    return {
+        "value": 1.0 / t32, "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
+        "sample": f"1 OS-EM slice-iteration ({ss} subsets) of one {n}^2 x {pa}-angle slice, FP32 compat "
+                  f"restatement of OSEM1/OSEM2 in C with OpenMP; {t32:.2f} s",
+        "fp64_oracle_seconds": t64,
+    }, rel(gpu_slice0, c32), rel(gpu_slice0, c64)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--pa", type=int, default=1200)
+    ap.add_argument("--ss", type=int, default=10)
+    ap.add_argument("--per-slice", type=int, default=4, help="slices timed in the per-slice-launch baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box here has one GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ctrecon import Context
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    B, n, pa, ss = args.batch, args.n, args.pa, args.ss
+    dev = torch.device("cuda:0")
+    log(f"generating {B} phantoms {n}^2 x {pa}")
+    ang, sinos = make_batch(B, n, pa, seed=2026, device=dev)
+    imgs = [torch.ones((n, n), dtype=torch.float32, device=dev) for _ in range(B)]
+    torch.cuda.synchronize()
+
+    ctx = Context(0)
+    ctx.set_geometry(n, n, pa, ss, angles_deg=ang)
+    tune = None
+    if not args.no_tune:
+        tune = ctx.autotune(n)
+        log("autotune (fp waves/block, bp rows/thread):", tune)
+
+    # first call: one iteration from the uniform start -> parity sample of slice 0
+    ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+    ctx.synchronize()
+    gpu_slice0 = imgs[0].cpu().numpy()[None]
+
+    # kernel timing pass (eager launches, HIP events on the library stream)
+    ctx.set_kernel_timing(True)
+    ctx.osem_batched(imgs, sinos, 1, use_graph=False)
+    ctx.synchronize()
+    fp_ms, bp_ms, nfp, nbp = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    log(f"FP kernel {fp_ms:.3f} ms/launch, BP kernel {bp_ms:.3f} ms/launch ({nfp}, {nbp} launches)")
+
+    for _ in range(args.warmup):
+        ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = B * args.steps / dt
+    log(f"batched: {value:.2f} slice-iters/s ({dt / args.steps * 1e3:.1f} ms per step)")
+
+    # per-slice launch baseline: own kernels, one slice per call, sync after every kernel
+    pctx = Context(0)
+    pctx.set_geometry(n, n, pa, ss, angles_deg=ang)
+    if tune:
+        pctx.autotune(n)
+    pctx.set_sync_each_kernel(True)
+    K = max(1, args.per_slice)
+    xs = [torch.ones((1, n, n), dtype=torch.float32, device=dev) for _ in range(K)]
+    ys = [sinos[i % B].view(1, pa, n) for i in range(K)]
+    pctx.osem_execution(xs[0], ys[0].clone(), 1)
+    pctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        pctx.osem_execution(xs[i], ys[i], 1)
+    pctx.synchronize()
+    dtp = time.perf_counter() - t0
+    per_slice = K / dtp
+    log(f"per-slice launch: {per_slice:.2f} slice-iters/s")
+
+    na = pa // ss
+    fp_flops = 14.0 * n * na * n * B            # 14 FLOP per bilinear sample, D = n samples per ray
+    fp_tflops = fp_flops / (fp_ms * 1e-3) / 1e12
+    step_flops = (14.0 * n * pa * n + 2.0 * n * n * pa) * B
+    hbm_bytes = (12.0 * ss * n * n + 12.0 * n * pa) * B
+    roofline = {
+        "bound": "valu", "achieved": round(fp_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(fp_tflops / PEAK_FP32_TFLOPS, 4),
+        "traffic": (FP_TRAFFIC_BYTES_PER_SLICE * B if FP_TRAFFIC_BYTES_PER_SLICE else None),
+        "kernel": "k_fp<FP_RATIO> (OSEM1: forward projection + ratio), one launch = one subset of every slice",
+        "fp_ms_per_launch": round(fp_ms, 4), "bp_ms_per_launch": round(bp_ms, 4),
+        "whole_step_tflops": round(step_flops * value / B / 1e12, 3),
+        "whole_step_hbm_frac": round(hbm_bytes * value / B / (PEAK_HBM_GBS * 1e9), 5),
+    }
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "slice-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {B} seeded 10-ellipse phantoms, analytic sinograms with Poisson noise at I0=1e4",
+        "config": {"workload": f"batched OS-EM, {B} slices of {n}x{n}, {pa} projections over [0,180), {n} bins, "
+                               f"{ss} subsets; step = one OS-EM iteration over the batch",
+                   "slices": B, "n": n, "pa": pa, "ss": ss, "autotune": tune},
+        "per_slice_launch": {"value": round(per_slice, 3), "unit": "slice-iters/s", "slices_timed": K},
+        "batched_over_per_slice": round(value / per_slice, 3),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("CPU baseline / oracle check on slice 0 ...")
+        cb, e32, e64 = cpu_baseline(n, pa, ss, ang, sinos[0].cpu().numpy(), gpu_slice0)
+        out["cpu_baseline"] = cb
+        out["relerr_vs_fp32_compat"] = e32
+        out["relerr_vs_fp64"] = e64
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    ctx.close()
+    pctx.close()
+
+
+if __name__ == "__main__":
+    main()

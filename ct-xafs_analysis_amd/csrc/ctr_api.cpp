@@ -1,0 +1,1209 @@
+// ctr_api.cpp -- C ABI of the MI355X CT projector hot path (declared in include/ctr.h).
+//
+// The execution drivers below follow the launch sequences of
+// CT_reconstruction/GPU.cpp (cited per function).  Where the reference copies
+// images between OpenCL image objects because OpenCL 1.2 cannot read and write
+// one image in a kernel, the update here is done in place (each
+// backprojection thread reads and writes only its own pixels).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ctr.h"
+#include "ctr_internal.h"
+
+using namespace ctr;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n)
+    {
+        if (n == 0) n = 4;
+        if (n <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    float *f() { return static_cast<float *>(p); }
+};
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// A set of slices with device descriptors and logical-block maps per grid kind.
+struct SliceSet {
+    std::vector<SliceDev> h;
+    std::vector<int> hmap[6];
+    DevBuf dsl, dmap[6];
+    int nb[6] = {0, 0, 0, 0, 0, 0};
+    int fp_wpb = 4, bp_rows = 4, na = 1;
+};
+
+struct Layout {
+    int nx, ny, px, ipitch, tpitch, rpitch;
+};
+
+Layout make_layout(int nx, int ny, int px)
+{
+    Layout l;
+    l.nx = nx; l.ny = ny; l.px = px;
+    l.ipitch = round_up(nx + 2 * PAD, 16);
+    l.tpitch = round_up(ny + 2 * PAD, 16);
+    l.rpitch = round_up(px + 4, 16);
+    return l;
+}
+
+size_t pimg_floats(const Layout &l) { return (size_t)(l.ny + 2 * PAD) * l.ipitch; }
+size_t pimgT_floats(const Layout &l) { return (size_t)(l.nx + 2 * PAD) * l.tpitch; }
+
+struct BatchPlan {
+    std::vector<ctr_slice> key;
+    int it_graph = -1;
+    SliceSet set;
+    DevBuf pimg, pimgT, rows;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchPlan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+}  // namespace
+
+struct ctr_ctx {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    bool sync_each = false;
+    std::string err;
+    bool has_geom = false;
+    ctr_geometry g{};
+    std::vector<float> ang;
+    std::vector<int> sub;
+    DevBuf cs, cs_corr, tw;
+    int tw_zp = 0;
+    // stack workspace
+    DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull;
+    SliceSet set_a, set_b;
+    // autotune table: n -> (fp waves per block, bp rows per thread)
+    std::map<int, std::pair<int, int>> tune;
+    std::unique_ptr<BatchPlan> plan;
+    // kernel timing
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double fp_ms = 0.0, bp_ms = 0.0;
+    int fp_n = 0, bp_n = 0;
+};
+
+namespace {
+
+ctr_status fail(ctr_ctx *c, ctr_status st, const char *fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return st;
+}
+
+ctr_status hipfail(ctr_ctx *c, const char *what, hipError_t e)
+{
+    return fail(c, e == hipErrorOutOfMemory ? CTR_OUT_OF_MEMORY : CTR_HIP_ERROR, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define HIPTRY(c, call)                                          \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return hipfail((c), #call, e_);    \
+    } while (0)
+
+// Called after every launch: reports launch errors, and mirrors GPU.cpp's
+// finish() after each enqueue when sync_each is set.
+ctr_status post(ctr_ctx *c, const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hipfail(c, what, e);
+    if (c->sync_each) {
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hipfail(c, what, e);
+    }
+    return CTR_OK;
+}
+
+#define POST(c, what)                            \
+    do {                                         \
+        ctr_status s_ = post((c), (what));       \
+        if (s_ != CTR_OK) return s_;             \
+    } while (0)
+
+#define TRY(expr)                                \
+    do {                                         \
+        ctr_status s_ = (expr);                  \
+        if (s_ != CTR_OK) return s_;             \
+    } while (0)
+
+std::pair<int, int> tuned(ctr_ctx *c, int n)
+{
+    auto it = c->tune.find(n);
+    if (it != c->tune.end()) return it->second;
+    return {4, 4};
+}
+
+// Fills `S` with one descriptor per slice and uploads descriptors and maps.
+ctr_status upload_set(ctr_ctx *c, SliceSet &S)
+{
+    const int n = (int)S.h.size();
+    for (int k = 0; k < 6; ++k) S.hmap[k].clear();
+    for (int s = 0; s < n; ++s) {
+        SliceDev &d = S.h[s];
+        const int cnt[6] = {fp_blocks(d.px, S.na, S.fp_wpb), bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny),
+                            pad_blocks(d.ny, d.ipitch), ew_blocks(d.nx, d.ny), 0};
+        for (int k = 0; k < 6; ++k) {
+            d.blk[k] = (int)S.hmap[k].size();
+            S.hmap[k].insert(S.hmap[k].end(), (size_t)cnt[k], s);
+        }
+    }
+    HIPTRY(c, S.dsl.ensure(sizeof(SliceDev) * std::max(n, 1)));
+    HIPTRY(c, hipMemcpyAsync(S.dsl.p, S.h.data(), sizeof(SliceDev) * n, hipMemcpyHostToDevice, c->stream));
+    for (int k = 0; k < 6; ++k) {
+        S.nb[k] = (int)S.hmap[k].size();
+        HIPTRY(c, S.dmap[k].ensure(sizeof(int) * std::max(S.nb[k], 1)));
+        if (S.nb[k])
+            HIPTRY(c, hipMemcpyAsync(S.dmap[k].p, S.hmap[k].data(), sizeof(int) * S.nb[k], hipMemcpyHostToDevice,
+                                     c->stream));
+    }
+    return CTR_OK;
+}
+
+Launch launch_of(ctr_ctx *c, SliceSet &S, int kind)
+{
+    Launch L;
+    L.sl = static_cast<const SliceDev *>(S.dsl.p);
+    L.bmap = static_cast<const int *>(S.dmap[kind].p);
+    L.cs = static_cast<const float2 *>(c->cs.p);
+    L.nblocks = S.nb[kind];
+    return L;
+}
+
+ctr_status need_geom(ctr_ctx *c)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    if (!c->has_geom) return fail(c, CTR_INVALID_ARGUMENT, "geometry not set (call ctr_set_geometry first)");
+    return CTR_OK;
+}
+
+// Binds a uniform stack of nz layers of the context geometry.
+struct StackBind {
+    float *pimg = nullptr, *pimgT = nullptr, *rows = nullptr;
+    const float *prj = nullptr, *src = nullptr;
+    float *dst = nullptr, *aux = nullptr;
+    int layer_base = 0;
+};
+
+ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
+{
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    const int na = g.pa / g.ss;
+    const auto t = tuned(c, std::max(g.nx, g.ny));
+    S.fp_wpb = t.first;
+    S.bp_rows = t.second;
+    S.na = na;
+    S.h.assign(nz, SliceDev{});
+    for (int z = 0; z < nz; ++z) {
+        SliceDev &d = S.h[z];
+        d.nx = g.nx; d.ny = g.ny; d.px = g.px;
+        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+        d.img = b.pimg ? b.pimg + (size_t)z * pimg_floats(l) : nullptr;
+        d.imgT = b.pimgT ? b.pimgT + (size_t)z * pimgT_floats(l) : nullptr;
+        d.rows = b.rows ? b.rows + (size_t)z * na * l.rpitch : nullptr;
+        d.prj = b.prj ? b.prj + (size_t)z * g.pa * g.px : nullptr;
+        d.src = b.src ? b.src + (size_t)z * g.nx * g.ny : nullptr;
+        d.dst = b.dst ? b.dst + (size_t)z * g.nx * g.ny : nullptr;
+        d.aux = b.aux ? b.aux + (size_t)z * g.nx * g.ny : nullptr;
+        d.layer = b.layer_base + z;
+    }
+    return upload_set(c, S);
+}
+
+// Device workspace for an nz-layer stack of the current geometry.
+ctr_status ensure_ws(ctr_ctx *c, int nz, bool second_image, int aux_planes)
+{
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    const int na = g.pa / g.ss;
+    HIPTRY(c, c->w_pimg.ensure(sizeof(float) * nz * pimg_floats(l)));
+    HIPTRY(c, c->w_pimgT.ensure(sizeof(float) * nz * pimgT_floats(l)));
+    HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)nz * std::max(na, g.pa) * l.rpitch));
+    if (second_image) {
+        HIPTRY(c, c->w_pimg2.ensure(sizeof(float) * nz * pimg_floats(l)));
+        HIPTRY(c, c->w_pimgT2.ensure(sizeof(float) * nz * pimgT_floats(l)));
+    }
+    const size_t plane = (size_t)nz * g.nx * g.ny * sizeof(float);
+    if (aux_planes > 0) HIPTRY(c, c->w_aux1.ensure(plane));
+    if (aux_planes > 1) HIPTRY(c, c->w_aux2.ensure(plane));
+    if (aux_planes > 2) HIPTRY(c, c->w_aux3.ensure(plane));
+    return CTR_OK;
+}
+
+ctr_status zero_rows(ctr_ctx *c, int nz, int nrows)
+{
+    const Layout l = make_layout(c->g.nx, c->g.ny, c->g.px);
+    launch_fill(c->w_rows.f(), (size_t)nz * nrows * l.rpitch, 0.0f, c->stream);
+    POST(c, "fill rows");
+    return CTR_OK;
+}
+
+ctr_status check_stack_args(ctr_ctx *c, const void *img, const void *prj, int dN)
+{
+    TRY(need_geom(c));
+    if (!img || !prj) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (dN < 1) return fail(c, CTR_INVALID_ARGUMENT, "dN must be >= 1 (got %d)", dN);
+    return CTR_OK;
+}
+
+ctr_status correct_in_place(ctr_ctx *c, float *prj, int dN)
+{
+    const ctr_geometry &g = c->g;
+    launch_sino_correction(prj, prj, static_cast<const float2 *>(c->cs_corr.p), dN, g.pa, g.px,
+                           g.correction_mode, 0.5f, 0.5f, c->stream);
+    POST(c, "sinogramCorrection");
+    return CTR_OK;
+}
+
+void time_begin(ctr_ctx *c, int which)
+{
+    if (c->timing) (void)hipEventRecord(c->ev[which * 2], c->stream);
+}
+
+void time_end(ctr_ctx *c, int which)
+{
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[which * 2 + 1], c->stream);
+    (void)hipEventSynchronize(c->ev[which * 2 + 1]);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev[which * 2], c->ev[which * 2 + 1]);
+    if (which == 0) { c->fp_ms += ms; c->fp_n++; }
+    else { c->bp_ms += ms; c->bp_n++; }
+}
+
+ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer, int geo_power)
+{
+    FpParams p{sub, c->g.ss, c->g.pa / c->g.ss, sub_from_layer, geo_power};
+    time_begin(c, 0);
+    launch_fp(var, S.fp_wpb, launch_of(c, S, G_FP), p, c->stream);
+    POST(c, "forward projection");
+    time_end(c, 0);
+    return CTR_OK;
+}
+
+ctr_status run_bp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer, float alpha, const float *L,
+                  int to_aux)
+{
+    BpParams p{sub, c->g.ss, c->g.pa / c->g.ss, c->g.pa, sub_from_layer, alpha, L, to_aux};
+    time_begin(c, 1);
+    launch_bp(var, S.bp_rows, launch_of(c, S, G_BP), p, c->stream);
+    POST(c, "backprojection");
+    time_end(c, 1);
+    return CTR_OK;
+}
+
+ctr_status run_kind(ctr_ctx *c, SliceSet &S, int kind)
+{
+    const Launch L = launch_of(c, S, kind);
+    if (kind == G_TR) launch_transpose(L, c->stream);
+    else if (kind == G_PAD) launch_pad_in(L, c->stream);
+    else launch_unpad_out(L, c->stream);
+    POST(c, kind == G_TR ? "transpose" : (kind == G_PAD ? "pad" : "unpad"));
+    return CTR_OK;
+}
+
+ctr_status angle_tables(ctr_ctx *c)
+{
+    const int pa = c->g.pa;
+    std::vector<float2> cs(pa), cc(pa);
+    for (int a = 0; a < pa; ++a) {
+        // angle_pr = angle[th]*PI/180.0f (double product, float result), cos/sin in float
+        const float ap = (float)((double)c->ang[a] * M_PI / 180.0);
+        cs[a] = make_float2(std::cos(ap), std::sin(ap));
+        // sinogramCorrection: angle[th]/180*PI (float division first)
+        const float ac = (float)((double)(c->ang[a] / 180.0f) * M_PI);
+        cc[a] = make_float2(std::cos(ac), 0.0f);
+    }
+    HIPTRY(c, c->cs.ensure(sizeof(float2) * pa));
+    HIPTRY(c, c->cs_corr.ensure(sizeof(float2) * pa));
+    HIPTRY(c, hipMemcpy(c->cs.p, cs.data(), sizeof(float2) * pa, hipMemcpyHostToDevice));
+    HIPTRY(c, hipMemcpy(c->cs_corr.p, cc.data(), sizeof(float2) * pa, hipMemcpyHostToDevice));
+    return CTR_OK;
+}
+
+ctr_status twiddles(ctr_ctx *c, int zp)
+{
+    if (c->tw_zp == zp) return CTR_OK;
+    std::vector<float2> tw(std::max(zp / 2, 1));
+    for (int l = 0; l < zp / 2; ++l) {
+        const double a = 2.0 * M_PI * (double)l / (double)zp;
+        tw[l] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    HIPTRY(c, c->tw.ensure(sizeof(float2) * tw.size()));
+    HIPTRY(c, hipMemcpy(c->tw.p, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+    c->tw_zp = zp;
+    return CTR_OK;
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+}  // namespace
+
+// ===========================================================================
+// context
+
+extern "C" const char *ctr_status_string(ctr_status s)
+{
+    switch (s) {
+    case CTR_OK: return "ok";
+    case CTR_INVALID_ARGUMENT: return "invalid argument";
+    case CTR_INVALID_GEOMETRY: return "invalid geometry";
+    case CTR_NO_DEVICE: return "no device";
+    case CTR_OUT_OF_MEMORY: return "out of memory";
+    case CTR_HIP_ERROR: return "HIP error";
+    case CTR_IO_ERROR: return "I/O error";
+    case CTR_NOT_SUPPORTED: return "not supported";
+    }
+    return "unknown status";
+}
+
+extern "C" ctr_status ctr_create(int device, ctr_ctx **out)
+{
+    if (!out) return CTR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CTR_NO_DEVICE;
+    if (device < 0 || device >= n) return CTR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return CTR_NO_DEVICE;
+    auto *c = new ctr_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CTR_HIP_ERROR;
+    }
+    c->stream = c->own;
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    *out = c;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_destroy(ctr_ctx *c)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->plan.reset();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t own = c->own;
+    delete c;  // frees device buffers
+    if (own) (void)hipStreamDestroy(own);
+    return CTR_OK;
+}
+
+extern "C" const char *ctr_last_error(const ctr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" ctr_status ctr_set_stream(ctr_ctx *c, void *s)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_synchronize(ctr_ctx *c)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_set_sync_each_kernel(ctr_ctx *c, int enable)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    c->sync_each = enable != 0;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_subset_order(int ss, int *sub)
+{
+    if (ss < 1 || !sub) return CTR_INVALID_ARGUMENT;
+    // GPU.cpp:564-582
+    int bits = 0;
+    for (int i = 0; i < 32; ++i) bits += (ss >> i) & 1;
+    if (bits == 1) {
+        int m1 = 0;
+        sub[m1++] = 0;
+        for (int i = ss, m2 = 1; i > 1; i /= 2, m2 *= 2)
+            for (int j = 0; j < m2; ++j) sub[m1++] = sub[j] + i / 2;
+    } else {
+        for (int i = 0; i < ss; ++i) sub[i] = i;
+    }
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_set_geometry(ctr_ctx *c, const ctr_geometry *g, const float *angles_deg, const int *sub)
+{
+    if (!c || !g || !angles_deg) return fail(c, CTR_INVALID_ARGUMENT, "null argument");
+    if (g->nx < 2 || g->ny < 2 || g->px < 2 || g->pa < 1 || g->ss < 1)
+        return fail(c, CTR_INVALID_GEOMETRY, "sizes must be positive (nx=%d ny=%d px=%d pa=%d ss=%d)", g->nx,
+                    g->ny, g->px, g->pa, g->ss);
+    if (g->nx > 16384 || g->ny > 16384 || g->px > 16384 || g->pa > (1 << 20))
+        return fail(c, CTR_INVALID_GEOMETRY, "sizes beyond the supported maximum (16384 px, 2^20 angles)");
+    if (g->ox != g->nx || g->oy != g->ny)
+        return fail(c, CTR_INVALID_GEOMETRY, "output ROI (%d x %d) must equal the grid (%d x %d)", g->ox, g->oy,
+                    g->nx, g->ny);
+    if (g->pa % g->ss != 0)
+        return fail(c, CTR_INVALID_GEOMETRY, "angle count %d is not a multiple of the subset count %d", g->pa,
+                    g->ss);
+    if (g->correction_mode < 0 || g->correction_mode > 4)
+        return fail(c, CTR_INVALID_GEOMETRY, "correction mode %d outside 0..4", g->correction_mode);
+    for (int a = 0; a < g->pa; ++a)
+        if (!std::isfinite(angles_deg[a])) return fail(c, CTR_INVALID_ARGUMENT, "angle %d is not finite", a);
+    std::vector<int> order(g->ss);
+    if (sub) {
+        std::vector<char> seen(g->ss, 0);
+        for (int k = 0; k < g->ss; ++k) {
+            if (sub[k] < 0 || sub[k] >= g->ss || seen[sub[k]])
+                return fail(c, CTR_INVALID_ARGUMENT, "subset order is not a permutation of 0..%d", g->ss - 1);
+            seen[sub[k]] = 1;
+            order[k] = sub[k];
+        }
+    } else {
+        ctr_subset_order(g->ss, order.data());
+    }
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    c->g = *g;
+    c->ang.assign(angles_deg, angles_deg + g->pa);
+    c->sub = order;
+    c->plan.reset();
+    c->tw_zp = 0;
+    TRY(angle_tables(c));
+    c->has_geom = true;
+    return CTR_OK;
+}
+
+// ===========================================================================
+// execution drivers
+
+// OSEM_execution, GPU.cpp:125-240: [sinogramCorrection] then for it, for k:
+// OSEM1(sub[k]) -> OSEM2(sub[k]) -> image update.
+extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int dN, int it, int prj_correction)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    const ctr_geometry &g = c->g;
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(ensure_ws(c, dN, false, 0));
+    TRY(zero_rows(c, dN, g.pa / g.ss));
+    if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
+    b.prj = prj; b.src = img; b.dst = img;
+    SliceSet &S = c->set_a;
+    TRY(build_stack(c, S, dN, b));
+    TRY(run_kind(c, S, G_PAD));
+    for (int i = 0; i < it; ++i)
+        for (int k = 0; k < g.ss; ++k) {
+            TRY(run_kind(c, S, G_TR));
+            TRY(run_fp(c, S, FP_RATIO, c->sub[k], 0, 0));
+            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0));
+        }
+    TRY(run_kind(c, S, G_EW));
+    return CTR_OK;
+}
+
+// FISTA_OSEM_execution, GPU.cpp:243-405.  L2norm is filled with 0.5
+// (GPU.cpp:326), so lambda = CSlambda.  First sub-iteration: ISTA3D of the
+// pre-update image replaces the OS-EM result (GPU.cpp:329-330, 369-377);
+// later: x <- prox of the OS-EM update (FISTA3D's x_new, GPU.cpp:337, 383).
+// FISTA3D's w and beta outputs never reach the returned image and are not
+// computed.
+extern "C" ctr_status ctr_fista_osem_execution(ctr_ctx *c, float *img, float *prj, int dN, int it,
+                                               int prj_correction)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    const ctr_geometry &g = c->g;
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(ensure_ws(c, dN, false, 1));
+    HIPTRY(c, c->w_l2.ensure(sizeof(float) * g.ss));
+    TRY(zero_rows(c, dN, g.pa / g.ss));
+    launch_fill(c->w_l2.f(), g.ss, 0.5f, c->stream);
+    POST(c, "fill L2norm");
+    if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
+    b.prj = prj; b.src = img; b.dst = img; b.aux = c->w_aux1.f();
+    SliceSet &S = c->set_a;
+    TRY(build_stack(c, S, dN, b));
+    TRY(run_kind(c, S, G_PAD));
+    bool first = true;
+    for (int i = 0; i < it; ++i)
+        for (int k = 0; k < g.ss; ++k) {
+            TRY(run_kind(c, S, G_TR));
+            TRY(run_fp(c, S, FP_RATIO, c->sub[k], 0, 0));
+            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 1));
+            // the prox reads an unpadded stack: the initial image (first step,
+            // still in the caller's buffer) or the OS-EM update (aux)
+            const float *v = first ? img : c->w_aux1.f();
+            launch_tv(first ? 0 : 1, nullptr, v, nullptr, nullptr, c->w_pimg.f(), nullptr, 1, l.ipitch, dN, g.nx,
+                      g.ny, c->w_l2.f(), c->sub[k], (double)g.cs_lambda, c->stream);
+            POST(c, first ? "ISTA3D" : "FISTA3D");
+            first = false;
+        }
+    TRY(run_kind(c, S, G_EW));
+    return CTR_OK;
+}
+
+// AART_execution, GPU.cpp:1088-1216: [sinogramCorrection when correctionMode >
+// 0], the image is replaced by backProjection of subset sub[0]
+// (GPU.cpp:1167-1176), then for it, for k: AART1 -> AART2.
+extern "C" ctr_status ctr_aart_execution(ctr_ctx *c, float *img, float *prj, int dN, int it)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    const ctr_geometry &g = c->g;
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(ensure_ws(c, dN, false, 0));
+    TRY(zero_rows(c, dN, g.pa / g.ss));
+    if (g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const float alpha = (g.mode == 3) ? g.wt2 : g.wt1;   // GPU.cpp:1161-1165
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
+    b.prj = prj; b.src = img; b.dst = img;
+    SliceSet &S = c->set_a;
+    TRY(build_stack(c, S, dN, b));
+    TRY(run_kind(c, S, G_PAD));   // zero borders (the interior is overwritten next)
+    TRY(run_bp(c, S, BP_INIT, c->sub[0], 0, 0.0f, nullptr, 0));
+    for (int i = 0; i < it; ++i)
+        for (int k = 0; k < g.ss; ++k) {
+            TRY(run_kind(c, S, G_TR));
+            TRY(run_fp(c, S, FP_RESID, c->sub[k], 0, 0));
+            TRY(run_bp(c, S, BP_AART2, c->sub[k], 0, alpha, nullptr, 0));
+        }
+    TRY(run_kind(c, S, G_EW));
+    return CTR_OK;
+}
+
+// FBP_execution, GPU.cpp:665-819: [sinogramCorrection], ramp filter of every
+// row (FBP.cl:34-160), backProjectionFBP (FBP.cl:164-188).
+extern "C" ctr_status ctr_fbp_execution(ctr_ctx *c, float *img, float *prj, int dN, int prj_correction)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    const ctr_geometry &g = c->g;
+    if (!is_pow2(g.zp) || g.zp < g.px || g.zp > 16384)
+        return fail(c, CTR_INVALID_GEOMETRY, "zero-padding size %d must be a power of two in [px, 16384]", g.zp);
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(twiddles(c, g.zp));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)dN * g.pa * l.rpitch));
+    launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
+    POST(c, "fill rows");
+    if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, g.zp, static_cast<const float2 *>(c->tw.p),
+                      c->stream);
+    POST(c, "FBP filter");
+    launch_fbp_bp(c->w_rows.f(), img, static_cast<const float2 *>(c->cs.p), dN, g.nx, g.ny, g.pa, g.px, l.rpitch,
+                  c->stream);
+    POST(c, "backProjectionFBP");
+    return CTR_OK;
+}
+
+// powerIteration, GPU.cpp:2079-2188: ss layers, layer s = subset s, filled
+// with 1.0; each trial powerIter1 -> powerIter2 -> imageL2AbsX/Y -> powerIter3.
+extern "C" ctr_status ctr_power_iteration(ctr_ctx *c, float *L2norm, int trials)
+{
+    TRY(need_geom(c));
+    if (!L2norm || trials < 1) return fail(c, CTR_INVALID_ARGUMENT, "need L2norm and trials >= 1");
+    const ctr_geometry &g = c->g;
+    const int S = g.ss;
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(ensure_ws(c, S, false, 2));
+    HIPTRY(c, c->w_partial.ensure(sizeof(float) * l2_partial_count(S)));
+    TRY(zero_rows(c, S, g.pa / g.ss));
+    const size_t plane = (size_t)g.nx * g.ny;
+    launch_fill(c->w_aux2.f(), plane * S, 1.0f, c->stream);
+    POST(c, "fill ones");
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
+    b.src = c->w_aux2.f(); b.aux = c->w_aux1.f();
+    SliceSet &St = c->set_a;
+    TRY(build_stack(c, St, S, b));
+    TRY(run_kind(c, St, G_PAD));
+    for (int t = 0; t < trials; ++t) {
+        TRY(run_kind(c, St, G_TR));
+        TRY(run_fp(c, St, FP_RAW, 0, 1, 1));
+        TRY(run_bp(c, St, BP_POWER, 0, 1, 0.0f, nullptr, 1));
+        launch_l2(c->w_aux1.f(), S, plane, c->w_partial.f(), L2norm, c->stream);
+        POST(c, "imageL2Abs");
+        launch_power3(c->w_aux1.f(), L2norm, c->w_pimg.f(), S, g.nx, g.ny, l.ipitch, c->stream);
+        POST(c, "powerIter3");
+    }
+    return CTR_OK;
+}
+
+// FISTA_execution, GPU.cpp:1755-1924, with the reference's argument bindings
+// (GPU.cpp:1827-1849, 1898-1899): the first step projects x0 (backProjection
+// of sub[0]) and ISTA3D writes x1 into the returned image; later steps project
+// w, and FISTA3D(x = x1, v, beta) updates w and beta, its x_new output being
+// overwritten by beta (both bound to beta_dest_img).
+extern "C" ctr_status ctr_fista_execution(ctr_ctx *c, const float *L2norm, float *img, float *prj, int dN, int it,
+                                          int prj_correction)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    if (!L2norm) return fail(c, CTR_INVALID_ARGUMENT, "null L2norm");
+    const ctr_geometry &g = c->g;
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(ensure_ws(c, dN, true, 2));
+    TRY(zero_rows(c, dN, g.pa / g.ss));
+    if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    const size_t plane = (size_t)g.nx * g.ny * dN;
+    launch_fill(c->w_aux2.f(), plane, 0.0f, c->stream);   // beta = 0 (GPU.cpp:1799)
+    POST(c, "fill beta");
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    // set_a: R (x) in pimg; set_b: W in pimg2.  aux1 = v, aux2 = beta.
+    StackBind br;
+    br.pimg = c->w_pimg.f(); br.pimgT = c->w_pimgT.f(); br.rows = c->w_rows.f();
+    br.prj = prj; br.src = img; br.dst = img; br.aux = c->w_aux1.f();
+    TRY(build_stack(c, c->set_a, dN, br));
+    StackBind bw = br;
+    bw.pimg = c->w_pimg2.f(); bw.pimgT = c->w_pimgT2.f();
+    TRY(build_stack(c, c->set_b, dN, bw));
+    // x0 = backProjection(y, sub[0]) into R; W = R (GPU.cpp:1852-1864)
+    TRY(run_kind(c, c->set_a, G_PAD));
+    TRY(run_bp(c, c->set_a, BP_INIT, c->sub[0], 0, 0.0f, nullptr, 0));
+    launch_copy(c->w_pimg.f(), c->w_pimg2.f(), (size_t)dN * pimg_floats(l), c->stream);
+    POST(c, "copy x0 -> w");
+    TRY(run_kind(c, c->set_a, G_EW));   // caller's buffer holds x0 (FISTA3D's x input later)
+    bool first = true;
+    for (int i = 0; i < it; ++i)
+        for (int k = 0; k < g.ss; ++k) {
+            SliceSet &S = first ? c->set_a : c->set_b;
+            TRY(run_kind(c, S, G_TR));
+            TRY(run_fp(c, S, FP_RESID, c->sub[k], 0, 0));
+            TRY(run_bp(c, S, BP_FISTA, c->sub[k], 0, g.wt1, L2norm, 1));
+            if (first) {
+                // ISTA3D(v) -> the returned image x1
+                launch_tv(0, nullptr, c->w_aux1.f(), nullptr, nullptr, img, nullptr, 0, l.ipitch, dN, g.nx, g.ny,
+                          L2norm, c->sub[k], (double)g.cs_lambda, c->stream);
+                POST(c, "ISTA3D");
+                first = false;
+            } else {
+                // FISTA3D(x = x1, v, beta) -> w (padded, next projection source), beta in place
+                launch_tv(1, img, c->w_aux1.f(), c->w_aux2.f(), c->w_pimg2.f(), nullptr, c->w_aux2.f(), 1,
+                          l.ipitch, dN, g.nx, g.ny, L2norm, c->sub[k], (double)g.cs_lambda, c->stream);
+                POST(c, "FISTA3D");
+            }
+        }
+    return CTR_OK;
+}
+
+// ===========================================================================
+// single operators
+
+namespace {
+
+ctr_status op_prologue(ctr_ctx *c, const void *a, const void *b, int dN)
+{
+    TRY(need_geom(c));
+    if (!a || !b) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (dN < 1) return fail(c, CTR_INVALID_ARGUMENT, "dN must be >= 1");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, CTR_HIP_ERROR, "hipSetDevice failed");
+    return CTR_OK;
+}
+
+ctr_status check_sub(ctr_ctx *c, int sub)
+{
+    if (sub < 0 || sub >= c->g.ss) return fail(c, CTR_INVALID_ARGUMENT, "subset %d outside 0..%d", sub, c->g.ss - 1);
+    return CTR_OK;
+}
+
+// pad `img` (may be null for zero) into the workspace and bind a stack set
+ctr_status op_stack(ctr_ctx *c, int dN, const float *img, const float *prj, float *aux, bool pad, int layer_base = 0)
+{
+    TRY(ensure_ws(c, dN, false, 0));
+    TRY(zero_rows(c, dN, std::max(c->g.pa / c->g.ss, 1)));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
+    b.prj = prj; b.src = img; b.aux = aux; b.layer_base = layer_base;
+    TRY(build_stack(c, c->set_a, dN, b));
+    if (pad) {
+        TRY(run_kind(c, c->set_a, G_PAD));
+        TRY(run_kind(c, c->set_a, G_TR));
+    }
+    return CTR_OK;
+}
+
+ctr_status op_fp(ctr_ctx *c, int var, const float *img, const float *prj, float *out_full, int dN, int sub)
+{
+    TRY(op_prologue(c, img, out_full, dN));
+    TRY(check_sub(c, sub));
+    TRY(op_stack(c, dN, img, prj, nullptr, true));
+    TRY(run_fp(c, c->set_a, var, sub, 0, 0));
+    const Layout l = make_layout(c->g.nx, c->g.ny, c->g.px);
+    launch_rows_to_full(c->w_rows.f(), out_full, dN, c->g.pa / c->g.ss, l.rpitch, c->g.pa, c->g.px, sub, c->g.ss,
+                        c->stream);
+    POST(c, "rows to full");
+    return CTR_OK;
+}
+
+ctr_status op_bp(ctr_ctx *c, int var, const float *img, const float *rows_full, float *dst, int dN, int sub,
+                 float alpha, const float *L)
+{
+    TRY(op_prologue(c, rows_full, dst, dN));
+    TRY(check_sub(c, sub));
+    const bool raw = (var == BP_INIT);
+    TRY(op_stack(c, dN, img, raw ? rows_full : nullptr, dst, img != nullptr));
+    if (!raw) {
+        const Layout l = make_layout(c->g.nx, c->g.ny, c->g.px);
+        launch_full_to_rows(rows_full, c->w_rows.f(), dN, c->g.pa / c->g.ss, l.rpitch, c->g.pa, c->g.px, sub,
+                            c->g.ss, c->stream);
+        POST(c, "full to rows");
+    }
+    TRY(run_bp(c, c->set_a, var, sub, 0, alpha, L, 1));
+    return CTR_OK;
+}
+
+}  // namespace
+
+extern "C" ctr_status ctr_op_osem1(ctr_ctx *c, const float *img, const float *prj, float *rprj, int dN, int sub)
+{
+    if (!prj) return fail(c, CTR_INVALID_ARGUMENT, "null prj");
+    return op_fp(c, FP_RATIO, img, prj, rprj, dN, sub);
+}
+
+extern "C" ctr_status ctr_op_aart1(ctr_ctx *c, const float *img, const float *prj, float *dprj, int dN, int sub)
+{
+    if (!prj) return fail(c, CTR_INVALID_ARGUMENT, "null prj");
+    return op_fp(c, FP_RESID, img, prj, dprj, dN, sub);
+}
+
+extern "C" ctr_status ctr_op_osem2(ctr_ctx *c, const float *img, const float *rprj, float *img_dst, int dN, int sub)
+{
+    if (!img) return fail(c, CTR_INVALID_ARGUMENT, "null img");
+    return op_bp(c, BP_OSEM2, img, rprj, img_dst, dN, sub, 0.0f, nullptr);
+}
+
+extern "C" ctr_status ctr_op_aart2(ctr_ctx *c, const float *img, const float *dprj, float *img_dst, int dN, int sub,
+                                   float alpha)
+{
+    if (!img) return fail(c, CTR_INVALID_ARGUMENT, "null img");
+    return op_bp(c, BP_AART2, img, dprj, img_dst, dN, sub, alpha, nullptr);
+}
+
+extern "C" ctr_status ctr_op_backprojection(ctr_ctx *c, const float *prj, float *img_dst, int dN, int sub)
+{
+    return op_bp(c, BP_INIT, nullptr, prj, img_dst, dN, sub, 0.0f, nullptr);
+}
+
+extern "C" ctr_status ctr_op_fista_backprojection(ctr_ctx *c, const float *img, const float *dprj, const float *L,
+                                                  float *img_dst, int dN, int sub, float alpha)
+{
+    if (!img || !L) return fail(c, CTR_INVALID_ARGUMENT, "null img or L2norm");
+    return op_bp(c, BP_FISTA, img, dprj, img_dst, dN, sub, alpha, L);
+}
+
+extern "C" ctr_status ctr_op_sinogram_correction(ctr_ctx *c, const float *src, float *dst, int dN, int mode)
+{
+    TRY(op_prologue(c, src, dst, dN));
+    if (mode < 0 || mode > 4) return fail(c, CTR_INVALID_ARGUMENT, "mode %d outside 0..4", mode);
+    launch_sino_correction(src, dst, static_cast<const float2 *>(c->cs_corr.p), dN, c->g.pa, c->g.px, mode, 0.5f,
+                           0.5f, c->stream);
+    POST(c, "sinogramCorrection");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_fbp_filter(ctr_ctx *c, const float *prj, float *fprj, int dN)
+{
+    TRY(op_prologue(c, prj, fprj, dN));
+    const ctr_geometry &g = c->g;
+    if (!is_pow2(g.zp) || g.zp < g.px || g.zp > 16384)
+        return fail(c, CTR_INVALID_GEOMETRY, "zero-padding size %d must be a power of two in [px, 16384]", g.zp);
+    TRY(twiddles(c, g.zp));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)dN * g.pa * l.rpitch));
+    launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
+    POST(c, "fill rows");
+    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, g.zp, static_cast<const float2 *>(c->tw.p),
+                      c->stream);
+    POST(c, "FBP filter");
+    // all rows are "subset 0 of 1": copy the padded rows back out
+    launch_rows_to_full(c->w_rows.f(), fprj, dN, g.pa, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    POST(c, "rows to full");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_fbp_backprojection(ctr_ctx *c, const float *fprj, float *img, int dN)
+{
+    TRY(op_prologue(c, fprj, img, dN));
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)dN * g.pa * l.rpitch));
+    launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
+    POST(c, "fill rows");
+    launch_full_to_rows(fprj, c->w_rows.f(), dN, g.pa, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    POST(c, "full to rows");
+    launch_fbp_bp(c->w_rows.f(), img, static_cast<const float2 *>(c->cs.p), dN, g.nx, g.ny, g.pa, g.px, l.rpitch,
+                  c->stream);
+    POST(c, "backProjectionFBP");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_power_iter1(ctr_ctx *c, const float *img, float *prjc)
+{
+    TRY(op_prologue(c, img, prjc, 1));
+    const ctr_geometry &g = c->g;
+    TRY(op_stack(c, g.ss, img, nullptr, nullptr, true));
+    TRY(run_fp(c, c->set_a, FP_RAW, 0, 1, 1));
+    // compact rows of layer s go to rows 0..na-1 of layer s (FISTA.cl:61)
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    launch_rows_to_full(c->w_rows.f(), prjc, g.ss, g.pa / g.ss, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    POST(c, "rows to full");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_power_iter2(ctr_ctx *c, const float *prjc, float *cnd)
+{
+    TRY(op_prologue(c, prjc, cnd, 1));
+    const ctr_geometry &g = c->g;
+    TRY(op_stack(c, g.ss, nullptr, nullptr, cnd, false));
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    launch_full_to_rows(prjc, c->w_rows.f(), g.ss, g.pa / g.ss, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    POST(c, "full to rows");
+    TRY(run_bp(c, c->set_a, BP_POWER, 0, 1, 0.0f, nullptr, 1));
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_ista3d(ctr_ctx *c, const float *v, float *x_new, int dN, int sub, const float *L)
+{
+    TRY(op_prologue(c, v, x_new, dN));
+    TRY(check_sub(c, sub));
+    if (!L) return fail(c, CTR_INVALID_ARGUMENT, "null L2norm");
+    launch_tv(0, nullptr, v, nullptr, nullptr, x_new, nullptr, 0, 0, dN, c->g.nx, c->g.ny, L, sub,
+              (double)c->g.cs_lambda, c->stream);
+    POST(c, "ISTA3D");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_op_fista3d(ctr_ctx *c, const float *x, const float *v, const float *b, float *w_new,
+                                     float *x_new, float *b_new, int dN, int sub, const float *L)
+{
+    TRY(op_prologue(c, x, v, dN));
+    TRY(check_sub(c, sub));
+    if (!b || !L) return fail(c, CTR_INVALID_ARGUMENT, "null beta or L2norm");
+    if (x_new && x_new == b_new) x_new = nullptr;   // reference write order: beta remains
+    launch_tv(1, x, v, b, w_new, x_new, b_new, 0, 0, dN, c->g.nx, c->g.ny, L, sub, (double)c->g.cs_lambda,
+              c->stream);
+    POST(c, "FISTA3D");
+    return CTR_OK;
+}
+
+// ===========================================================================
+// batched uneven stacks
+
+namespace {
+
+bool same_key(const std::vector<ctr_slice> &a, const ctr_slice *b, int n)
+{
+    if ((int)a.size() != n) return false;
+    for (int i = 0; i < n; ++i)
+        if (a[i].n != b[i].n || a[i].img != b[i].img || a[i].prj != b[i].prj) return false;
+    return true;
+}
+
+// One OS-EM iteration over the whole plan: per subset a transpose, a
+// projection and a backprojection launch, each covering every slice.
+ctr_status batched_iteration(ctr_ctx *c, BatchPlan &P)
+{
+    for (int k = 0; k < c->g.ss; ++k) {
+        TRY(run_kind(c, P.set, G_TR));
+        TRY(run_fp(c, P.set, FP_RATIO, c->sub[k], 0, 0));
+        TRY(run_bp(c, P.set, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0));
+    }
+    return CTR_OK;
+}
+
+}  // namespace
+
+extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int count, int it, int use_graph)
+{
+    TRY(need_geom(c));
+    if (!slices || count < 1 || it < 0) return fail(c, CTR_INVALID_ARGUMENT, "need slices, count >= 1, it >= 0");
+    for (int i = 0; i < count; ++i)
+        if (slices[i].n < 2 || slices[i].n > 16384 || !slices[i].img || !slices[i].prj)
+            return fail(c, CTR_INVALID_ARGUMENT, "slice %d: side %d or null buffer", i, slices[i].n);
+    HIPTRY(c, hipSetDevice(c->device));
+    const int na = c->g.pa / c->g.ss;
+    if (!c->plan || !same_key(c->plan->key, slices, count)) {
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+        c->plan.reset(new BatchPlan());
+        BatchPlan &P = *c->plan;
+        P.key.assign(slices, slices + count);
+        // largest slices first: their blocks start earliest, evening out the tail
+        std::vector<int> order(count);
+        for (int i = 0; i < count; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return slices[a].n > slices[b].n; });
+        size_t fi = 0, ft = 0, fr = 0;
+        std::vector<Layout> lay(count);
+        for (int i = 0; i < count; ++i) {
+            lay[i] = make_layout(slices[i].n, slices[i].n, slices[i].n);
+            fi += pimg_floats(lay[i]);
+            ft += pimgT_floats(lay[i]);
+            fr += (size_t)na * lay[i].rpitch;
+        }
+        HIPTRY(c, P.pimg.ensure(sizeof(float) * fi));
+        HIPTRY(c, P.pimgT.ensure(sizeof(float) * ft));
+        HIPTRY(c, P.rows.ensure(sizeof(float) * fr));
+        launch_fill(P.rows.f(), fr, 0.0f, c->stream);
+        POST(c, "fill rows");
+        const int n0 = slices[order[0]].n;
+        const auto t = tuned(c, n0);
+        P.set.fp_wpb = t.first;
+        P.set.bp_rows = t.second;
+        P.set.na = na;
+        P.set.h.assign(count, SliceDev{});
+        size_t oi = 0, ot = 0, orr = 0;
+        for (int r = 0; r < count; ++r) {
+            const int i = order[r];
+            const Layout &l = lay[i];
+            SliceDev &d = P.set.h[r];
+            d.nx = d.ny = d.px = slices[i].n;
+            d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+            d.img = P.pimg.f() + oi;
+            d.imgT = P.pimgT.f() + ot;
+            d.rows = P.rows.f() + orr;
+            d.prj = slices[i].prj;
+            d.src = slices[i].img;
+            d.dst = slices[i].img;
+            d.layer = r;
+            oi += pimg_floats(l);
+            ot += pimgT_floats(l);
+            orr += (size_t)na * l.rpitch;
+        }
+        TRY(upload_set(c, P.set));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+    }
+    BatchPlan &P = *c->plan;
+    TRY(run_kind(c, P.set, G_PAD));
+    const bool graph = use_graph && !c->sync_each && !c->timing;
+    if (graph && it > 0 && !P.exec) {
+        HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        ctr_status st = batched_iteration(c, P);
+        hipGraph_t gph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &gph);
+        if (st != CTR_OK) {
+            if (gph) (void)hipGraphDestroy(gph);
+            return st;
+        }
+        if (e != hipSuccess) return hipfail(c, "hipStreamEndCapture", e);
+        P.graph = gph;
+        HIPTRY(c, hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < it; ++i) {
+        if (graph) {
+            HIPTRY(c, hipGraphLaunch(P.exec, c->stream));
+        } else {
+            TRY(batched_iteration(c, P));
+        }
+    }
+    TRY(run_kind(c, P.set, G_EW));
+    return CTR_OK;
+}
+
+// ===========================================================================
+// timing + autotune
+
+extern "C" ctr_status ctr_set_kernel_timing(ctr_ctx *c, int enable)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    c->timing = enable != 0;
+    c->fp_ms = c->bp_ms = 0.0;
+    c->fp_n = c->bp_n = 0;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_kernel_timing(ctr_ctx *c, double *fp_ms, double *bp_ms, int *fp_n, int *bp_n)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    if (fp_ms) *fp_ms = c->fp_n ? c->fp_ms / c->fp_n : 0.0;
+    if (bp_ms) *bp_ms = c->bp_n ? c->bp_ms / c->bp_n : 0.0;
+    if (fp_n) *fp_n = c->fp_n;
+    if (bp_n) *bp_n = c->bp_n;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_waves, int *bp_rows)
+{
+    TRY(need_geom(c));
+    if (n < 2 || n > 16384) return fail(c, CTR_INVALID_ARGUMENT, "side %d out of range", n);
+    HIPTRY(c, hipSetDevice(c->device));
+    const int na = c->g.pa / c->g.ss;
+    const int reps = 3;
+    // a synthetic slice: disk image, ones sinogram
+    const Layout l = make_layout(n, n, n);
+    DevBuf pimg, pimgT, rows, prj;
+    HIPTRY(c, pimg.ensure(sizeof(float) * pimg_floats(l)));
+    HIPTRY(c, pimgT.ensure(sizeof(float) * pimgT_floats(l)));
+    HIPTRY(c, rows.ensure(sizeof(float) * (size_t)na * l.rpitch));
+    HIPTRY(c, prj.ensure(sizeof(float) * (size_t)c->g.pa * n));
+    launch_fill(pimg.f(), pimg_floats(l), 1.0f, c->stream);
+    launch_fill(pimgT.f(), pimgT_floats(l), 1.0f, c->stream);
+    launch_fill(rows.f(), (size_t)na * l.rpitch, 0.0f, c->stream);
+    launch_fill(prj.f(), (size_t)c->g.pa * n, 1.0f, c->stream);
+    POST(c, "autotune fill");
+    SliceSet S;
+    S.na = na;
+    S.h.assign(1, SliceDev{});
+    SliceDev &d = S.h[0];
+    d.nx = d.ny = d.px = n;
+    d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+    d.img = pimg.f(); d.imgT = pimgT.f(); d.rows = rows.f(); d.prj = prj.f();
+    hipEvent_t e0, e1;
+    HIPTRY(c, hipEventCreate(&e0));
+    HIPTRY(c, hipEventCreate(&e1));
+    auto timeit = [&](auto &&fn) -> double {
+        fn();
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) fn();
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms / reps;
+    };
+    int best_w = 4, best_r = 4;
+    double best = 1e30;
+    const int wopts[3] = {2, 4, 8};
+    for (int w : wopts) {
+        S.fp_wpb = w;
+        S.bp_rows = 4;
+        TRY(upload_set(c, S));
+        FpParams p{0, c->g.ss, na, 0, 0};
+        const double ms = timeit([&] { launch_fp(FP_RATIO, w, launch_of(c, S, G_FP), p, c->stream); });
+        if (ms < best) { best = ms; best_w = w; }
+    }
+    best = 1e30;
+    const int ropts[3] = {2, 4, 8};
+    for (int r : ropts) {
+        S.fp_wpb = best_w;
+        S.bp_rows = r;
+        TRY(upload_set(c, S));
+        BpParams p{0, c->g.ss, na, c->g.pa, 0, 0.0f, nullptr, 0};
+        const double ms = timeit([&] { launch_bp(BP_OSEM2, r, launch_of(c, S, G_BP), p, c->stream); });
+        if (ms < best) { best = ms; best_r = r; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPTRY(c, hipGetLastError());
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    c->tune[n] = {best_w, best_r};
+    c->plan.reset();   // plans pick up the new choice on rebuild
+    if (fp_waves) *fp_waves = best_w;
+    if (bp_rows) *bp_rows = best_r;
+    return CTR_OK;
+}
+
+// ===========================================================================
+// differentiable alignment model
+
+namespace {
+
+ctr_status align_pad(ctr_ctx *c, const float *img, int dN)
+{
+    const ctr_geometry &g = c->g;
+    TRY(ensure_ws(c, dN, false, 0));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    StackBind b;
+    b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f(); b.src = img;
+    TRY(build_stack(c, c->set_a, dN, b));
+    TRY(run_kind(c, c->set_a, G_PAD));
+    TRY(run_kind(c, c->set_a, G_TR));
+    (void)g;
+    return CTR_OK;
+}
+
+}  // namespace
+
+extern "C" ctr_status ctr_align_forward(ctr_ctx *c, const float *img, const float *shifts, float center, float *prj,
+                                        int dN)
+{
+    TRY(op_prologue(c, img, prj, dN));
+    if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
+    TRY(align_pad(c, img, dN));
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    launch_align_forward(c->w_pimg.f(), c->w_pimgT.f(), l.ipitch, l.tpitch, static_cast<const float2 *>(c->cs.p),
+                         shifts, center, prj, dN, g.nx, g.ny, g.pa, g.px, c->stream);
+    POST(c, "align forward");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_align_adjoint(ctr_ctx *c, const float *prj, const float *shifts, float center, float *img,
+                                        int dN)
+{
+    TRY(op_prologue(c, prj, img, dN));
+    if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
+    const ctr_geometry &g = c->g;
+    launch_align_adjoint(prj, static_cast<const float2 *>(c->cs.p), shifts, center, img, dN, g.nx, g.ny, g.pa, g.px,
+                         c->stream);
+    POST(c, "align adjoint");
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_align_grad_shift(ctr_ctx *c, const float *img, const float *gprj, const float *shifts,
+                                           float center, float *gshift, int dN)
+{
+    TRY(op_prologue(c, img, gprj, dN));
+    if (!gshift) return fail(c, CTR_INVALID_ARGUMENT, "null gshift");
+    if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
+    TRY(align_pad(c, img, dN));
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    const size_t parts = (size_t)g.pa * dN * ((g.px + 63) / 64);
+    HIPTRY(c, c->w_partial.ensure(sizeof(float) * parts));
+    launch_align_grad(c->w_pimg.f(), c->w_pimgT.f(), l.ipitch, l.tpitch, static_cast<const float2 *>(c->cs.p), gprj,
+                      shifts, center, c->w_partial.f(), gshift, dN, g.nx, g.ny, g.pa, g.px, c->stream);
+    POST(c, "align grad");
+    return CTR_OK;
+}

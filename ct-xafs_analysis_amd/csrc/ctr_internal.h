@@ -1,0 +1,105 @@
+// ctr_internal.h -- structures shared by the gfx950 kernels and the C ABI layer.
+//
+// Internal data layout in HBM (one "slice" = one reconstruction layer):
+//   pimg   zero-padded image, PAD texels of zeros on every side, row pitch
+//          ipitch floats, (ny + 2 PAD) rows.  The padding lets the bilinear
+//          gather read all four taps without bounds tests: coordinates are
+//          clamped into the padded frame, and any clamped sample lands on
+//          zeros, which is exactly CLK_ADDRESS_CLAMP's border value.
+//   pimgT  the same image transposed (pitch tpitch, nx + 2 PAD rows).  Rays
+//          steeper than 45 degrees read it so that the 64 lanes of a wave,
+//          which hold 64 neighbouring detector bins, touch consecutive
+//          addresses.
+//   rows   compact subset rows [pa/ss][rpitch]: entry 1 + X holds bin X; every
+//          other entry stays zero, so nearest/linear bin reads clamped to
+//          [-1, px] land on zeros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ctr {
+
+constexpr int PAD = 2;
+
+struct SliceDev {
+    float *img;          // padded image (pitch ipitch, ny + 2 PAD rows)
+    float *imgT;         // padded transposed image (pitch tpitch, nx + 2 PAD rows)
+    const float *prj;    // sinogram [pa][px], unpadded
+    float *rows;         // compact subset rows [pa/ss][rpitch]
+    const float *src;    // unpadded [ny][nx] source of pad_in
+    float *dst;          // unpadded [ny][nx] destination of unpad_out
+    float *aux;          // unpadded [ny][nx] output plane of to_aux backprojections
+    int nx, ny, px;
+    int ipitch, tpitch, rpitch;
+    int layer;           // index in its stack (subset index for the power iteration)
+    int blk[6];          // first logical block of this slice for each grid kind
+};
+
+enum GridKind { G_FP = 0, G_BP = 1, G_TR = 2, G_PAD = 3, G_EW = 4, G_SPARE = 5 };
+
+// Forward-projection epilogues.
+enum FpVar { FP_RATIO = 0, FP_RESID = 1, FP_RAW = 2 };
+// Backprojection variants (kernel + epilogue).
+enum BpVar { BP_OSEM2 = 0, BP_AART2 = 1, BP_INIT = 2, BP_FISTA = 3, BP_POWER = 4 };
+
+struct Launch {
+    const SliceDev *sl;   // device descriptor array
+    const int *bmap;      // device logical-block -> slice map for this grid kind
+    const float2 *cs;     // device [pa] (cos, sin) of the float angle
+    int nblocks;
+};
+
+struct FpParams {
+    int sub, ss, na, sub_from_layer, geo_power;
+};
+
+struct BpParams {
+    int sub, ss, na, pa, sub_from_layer;
+    float alpha;
+    const float *L;       // L2norm (FISTA backprojection) or null
+    int to_aux;           // write the result to aux instead of the padded image
+};
+
+// ---- launchers (ctr_kernels.hip) ----
+void launch_fp(int var, int wpb, const Launch &L, const FpParams &p, hipStream_t stream);
+void launch_bp(int var, int rows, const Launch &L, const BpParams &p, hipStream_t stream);
+void launch_transpose(const Launch &L, hipStream_t stream);
+void launch_pad_in(const Launch &L, hipStream_t stream);
+void launch_unpad_out(const Launch &L, hipStream_t stream);
+void launch_fill(float *dst, size_t n, float value, hipStream_t stream);
+void launch_copy(const float *src, float *dst, size_t n, hipStream_t stream);
+void launch_sino_correction(const float *src, float *dst, const float2 *cs_corr, int nz, int pa,
+                            int px, int mode, float a, float b, hipStream_t stream);
+void launch_tv(int fista, const float *x, const float *v, const float *b, float *w_new, float *x_new,
+               float *b_new, int out_padded, int ipitch, int nz, int nx, int ny, const float *L,
+               int sub, double lambda_fista, hipStream_t stream);
+void launch_l2(const float *src, int nz, size_t n, float *partial, float *l2, hipStream_t stream);
+void launch_power3(const float *cnd, const float *l2, float *pimg, int nz, int nx, int ny, int ipitch,
+                   hipStream_t stream);
+void launch_fbp_filter(const float *prj, float *frows, int nz, int pa, int px, int rpitch, int zp,
+                       const float2 *tw, hipStream_t stream);
+void launch_fbp_bp(const float *frows, float *img, const float2 *cs, int nz, int nx, int ny, int pa,
+                   int px, int rpitch, hipStream_t stream);
+void launch_rows_to_full(const float *rows, float *full, int nz, int na, int rpitch, int pa, int px,
+                         int sub, int ss, hipStream_t stream);
+void launch_full_to_rows(const float *full, float *rows, int nz, int na, int rpitch, int pa, int px,
+                         int sub, int ss, hipStream_t stream);
+void launch_align_forward(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,
+                          const float *shifts, float center, float *prj, int nz, int nx, int ny, int pa,
+                          int px, hipStream_t stream);
+void launch_align_adjoint(const float *prj, const float2 *cs, const float *shifts, float center, float *img,
+                          int nz, int nx, int ny, int pa, int px, hipStream_t stream);
+void launch_align_grad(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,
+                       const float *gprj, const float *shifts, float center, float *partial, float *gshift,
+                       int nz, int nx, int ny, int pa, int px, hipStream_t stream);
+
+// blocks per slice for each grid kind
+int fp_blocks(int px, int na, int wpb);
+int bp_blocks(int nx, int ny, int rows);
+int tr_blocks(int nx, int ny);
+int pad_blocks(int ny, int ipitch);
+int ew_blocks(int nx, int ny);
+size_t l2_partial_count(int nz);
+
+}  // namespace ctr

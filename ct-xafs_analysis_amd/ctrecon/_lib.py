@@ -1,0 +1,250 @@
+"""ctypes binding of the C ABI in include/ctr.h (libctr_hip.so).
+
+torch is only plumbing here: device tensors provide device memory and the
+stream; every computation runs in the hand-written gfx950 kernels of the
+library.  There is no fallback path: if the library or a GPU is missing the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libctr_hip.so")
+
+CTR_OK = 0
+_STATUS = {0: "ok", 1: "invalid argument", 2: "invalid geometry", 3: "no device", 4: "out of memory",
+           5: "HIP error", 6: "I/O error", 7: "not supported"}
+
+
+class CtrError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Geometry(C.Structure):
+    """ctr_geometry (include/ctr.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("ox", C.c_int), ("oy", C.c_int), ("px", C.c_int),
+                ("pa", C.c_int), ("ss", C.c_int), ("zp", C.c_int), ("mode", C.c_int),
+                ("correction_mode", C.c_int), ("wt1", C.c_float), ("wt2", C.c_float), ("cs_lambda", C.c_float)]
+
+
+class Slice(C.Structure):
+    """ctr_slice (include/ctr.h)."""
+    _fields_ = [("n", C.c_int), ("img", C.c_void_p), ("prj", C.c_void_p)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_SIGS = {
+    "ctr_create": [_I, C.POINTER(_P)],
+    "ctr_destroy": [_P],
+    "ctr_set_stream": [_P, _P],
+    "ctr_synchronize": [_P],
+    "ctr_set_sync_each_kernel": [_P, _I],
+    "ctr_set_geometry": [_P, C.POINTER(Geometry), _P, _P],
+    "ctr_subset_order": [_I, _P],
+    "ctr_osem_execution": [_P, _P, _P, _I, _I, _I],
+    "ctr_fista_osem_execution": [_P, _P, _P, _I, _I, _I],
+    "ctr_aart_execution": [_P, _P, _P, _I, _I],
+    "ctr_fbp_execution": [_P, _P, _P, _I, _I],
+    "ctr_power_iteration": [_P, _P, _I],
+    "ctr_fista_execution": [_P, _P, _P, _P, _I, _I, _I],
+    "ctr_op_osem1": [_P, _P, _P, _P, _I, _I],
+    "ctr_op_osem2": [_P, _P, _P, _P, _I, _I],
+    "ctr_op_aart1": [_P, _P, _P, _P, _I, _I],
+    "ctr_op_aart2": [_P, _P, _P, _P, _I, _I, _F],
+    "ctr_op_backprojection": [_P, _P, _P, _I, _I],
+    "ctr_op_sinogram_correction": [_P, _P, _P, _I, _I],
+    "ctr_op_fbp_filter": [_P, _P, _P, _I],
+    "ctr_op_fbp_backprojection": [_P, _P, _P, _I],
+    "ctr_op_power_iter1": [_P, _P, _P],
+    "ctr_op_power_iter2": [_P, _P, _P],
+    "ctr_op_fista_backprojection": [_P, _P, _P, _P, _P, _I, _I, _F],
+    "ctr_op_ista3d": [_P, _P, _P, _I, _I, _P],
+    "ctr_op_fista3d": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P],
+    "ctr_osem_batched": [_P, C.POINTER(Slice), _I, _I, _I],
+    "ctr_autotune": [_P, _I, C.POINTER(_I), C.POINTER(_I)],
+    "ctr_set_kernel_timing": [_P, _I],
+    "ctr_kernel_timing": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)],
+    "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
+    "ctr_align_adjoint": [_P, _P, _P, _F, _P, _I],
+    "ctr_align_grad_shift": [_P, _P, _P, _P, _F, _P, _I],
+}
+
+EXPORTED = tuple(list(_SIGS) + ["ctr_last_error", "ctr_status_string"])
+
+
+def load():
+    """Loads libctr_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or `make -C ct-xafs_analysis_amd`")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.ctr_last_error.argtypes = [_P]
+    lib.ctr_last_error.restype = C.c_char_p
+    lib.ctr_status_string.argtypes = [C.c_int]
+    lib.ctr_status_string.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def subset_order(ss: int) -> np.ndarray:
+    sub = np.zeros(ss, dtype=np.int32)
+    st = load().ctr_subset_order(ss, sub.ctypes.data_as(_P))
+    if st != CTR_OK:
+        raise CtrError(st, "ctr_subset_order")
+    return sub
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor on the GPU")
+    if not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+        raise ValueError("expected a contiguous float32 GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One library context (device + stream).  Methods take float32 GPU tensors
+    and run on the context stream; call synchronize() before reading results
+    from other streams."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = False):
+        import torch
+        self._torch = torch
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.ctr_create(device, C.byref(h))
+        if st != CTR_OK:
+            raise CtrError(st, f"ctr_create(device={device})")
+        self.h = h
+        self.geom = None
+        if use_torch_stream:
+            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+    def _chk(self, st, what):
+        if st != CTR_OK:
+            msg = self.lib.ctr_last_error(self.h).decode()
+            raise CtrError(st, f"{what}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ctr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        self._chk(self.lib.ctr_set_stream(self.h, C.c_void_p(stream_handle)), "ctr_set_stream")
+
+    def synchronize(self):
+        self._chk(self.lib.ctr_synchronize(self.h), "ctr_synchronize")
+
+    def set_sync_each_kernel(self, on: bool):
+        self._chk(self.lib.ctr_set_sync_each_kernel(self.h, int(bool(on))), "ctr_set_sync_each_kernel")
+
+    def set_geometry(self, nx, px, pa, ss=1, *, ny=None, zp=None, mode=6, correction_mode=0, wt1=1.0, wt2=0.01,
+                     cs_lambda=1e-4, angles_deg=None, sub=None):
+        ny = nx if ny is None else ny
+        if zp is None:
+            zp = 1
+            while zp < 2 * px:
+                zp *= 2
+        g = Geometry(nx, ny, nx, ny, px, pa, ss, zp, mode, correction_mode, wt1, wt2, cs_lambda)
+        ang = np.ascontiguousarray(angles_deg, dtype=np.float32)
+        if ang.shape != (pa,):
+            raise ValueError("angles_deg must hold pa values")
+        subp = None
+        if sub is not None:
+            sub = np.ascontiguousarray(sub, dtype=np.int32)
+            subp = sub.ctypes.data_as(_P)
+        self._chk(self.lib.ctr_set_geometry(self.h, C.byref(g), ang.ctypes.data_as(_P), subp), "ctr_set_geometry")
+        self.geom = g
+        return g
+
+    # ---- drivers ----
+    def osem_execution(self, img, prj, it, prj_correction=True):
+        self._chk(self.lib.ctr_osem_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction)),
+                  "ctr_osem_execution")
+
+    def fista_osem_execution(self, img, prj, it, prj_correction=True):
+        self._chk(self.lib.ctr_fista_osem_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it,
+                                                    int(prj_correction)), "ctr_fista_osem_execution")
+
+    def aart_execution(self, img, prj, it):
+        self._chk(self.lib.ctr_aart_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it), "ctr_aart_execution")
+
+    def fbp_execution(self, img, prj, prj_correction=True):
+        self._chk(self.lib.ctr_fbp_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction)),
+                  "ctr_fbp_execution")
+
+    def power_iteration(self, L, trials=30):
+        self._chk(self.lib.ctr_power_iteration(self.h, _ptr(L), trials), "ctr_power_iteration")
+
+    def fista_execution(self, L, img, prj, it, prj_correction=True):
+        self._chk(self.lib.ctr_fista_execution(self.h, _ptr(L), _ptr(img), _ptr(prj), img.shape[0], it,
+                                               int(prj_correction)), "ctr_fista_execution")
+
+    # ---- single operators ----
+    def op(self, name, *args):
+        f = getattr(self.lib, "ctr_op_" + name)
+        conv = [(_ptr(a) if (a is None or isinstance(a, self._torch.Tensor)) else a) for a in args]
+        self._chk(f(self.h, *conv), "ctr_op_" + name)
+
+    # ---- batched ----
+    def osem_batched(self, imgs, prjs, it, use_graph=True):
+        n = len(imgs)
+        arr = (Slice * n)()
+        for i, (im, pr) in enumerate(zip(imgs, prjs)):
+            arr[i] = Slice(int(im.shape[-1]), _ptr(im).value, _ptr(pr).value)
+        self._keep = (imgs, prjs, arr)
+        self._chk(self.lib.ctr_osem_batched(self.h, arr, n, it, int(use_graph)), "ctr_osem_batched")
+
+    def autotune(self, n):
+        w, r = C.c_int(), C.c_int()
+        self._chk(self.lib.ctr_autotune(self.h, n, C.byref(w), C.byref(r)), "ctr_autotune")
+        return w.value, r.value
+
+    def set_kernel_timing(self, on):
+        self._chk(self.lib.ctr_set_kernel_timing(self.h, int(bool(on))), "ctr_set_kernel_timing")
+
+    def kernel_timing(self):
+        a, b, na, nb = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        self._chk(self.lib.ctr_kernel_timing(self.h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb)),
+                  "ctr_kernel_timing")
+        return a.value, b.value, na.value, nb.value
+
+    # ---- differentiable model ----
+    def align_forward(self, img, shifts, center, prj):
+        self._chk(self.lib.ctr_align_forward(self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj),
+                                             img.shape[0]), "ctr_align_forward")
+
+    def align_adjoint(self, prj, shifts, center, img):
+        self._chk(self.lib.ctr_align_adjoint(self.h, _ptr(prj), _ptr(shifts), float(center), _ptr(img),
+                                             img.shape[0]), "ctr_align_adjoint")
+
+    def align_grad_shift(self, img, gprj, shifts, center, gshift):
+        self._chk(self.lib.ctr_align_grad_shift(self.h, _ptr(img), _ptr(gprj), _ptr(shifts), float(center),
+                                                _ptr(gshift), img.shape[0]), "ctr_align_grad_shift")

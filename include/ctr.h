@@ -1,0 +1,203 @@
+/*
+ * ctr.h -- C ABI of the MI355X CT reconstruction hot path.
+ *
+ * The library (ct-xafs_analysis_amd/libctr_hip.so) replaces the OpenCL queue
+ * that CT_reconstruction/GPU.cpp drives: the projector / backprojector kernels
+ * of OSEM.cl, AART.cl, FBP.cl, FISTA.cl and reconstShare.cl and the execution
+ * drivers that launch them.  Every entry point takes plain C types.
+ *
+ * Conventions
+ *  - Buffers named reconst_img / prj_img / L2norm / img / prj are DEVICE
+ *    pointers (hipMalloc'd or any HIP-visible device allocation), borrowed for
+ *    the duration of the call and used on the context's stream.
+ *  - Image stacks are [dN][ny][nx] float32, sinogram stacks [dN][pa][px]
+ *    float32, row-major with x fastest: the layouts of the reference's
+ *    Image2DArray objects (GPU.cpp:422-423) and of its raw files.
+ *  - Calls are asynchronous on the context stream unless stated otherwise;
+ *    ctr_synchronize() waits.  A context is thread-compatible, not
+ *    thread-safe: use one context per host thread.
+ *  - Errors are returned as ctr_status; ctr_last_error() gives the message.
+ *    The library never exits or aborts the caller's process.
+ */
+#ifndef CTR_H
+#define CTR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ctr_ctx ctr_ctx;
+
+typedef enum ctr_status {
+    CTR_OK = 0,
+    CTR_INVALID_ARGUMENT = 1,
+    CTR_INVALID_GEOMETRY = 2,
+    CTR_NO_DEVICE = 3,
+    CTR_OUT_OF_MEMORY = 4,
+    CTR_HIP_ERROR = 5,
+    CTR_IO_ERROR = 6,
+    CTR_NOT_SUPPORTED = 7
+} ctr_status;
+
+/* Reconstruction geometry and algorithm constants.  Replaces the globals of
+ * CT_reconstruction.hpp:110-135 that the reference compiles into each kernel
+ * program with -D (GPU.cpp:538-547, 948-954, 1336-1343, 2051-2059). */
+typedef struct ctr_geometry {
+    int nx, ny;           /* g_nx, g_ny: reconstruction grid */
+    int ox, oy;           /* g_ox, g_oy: output ROI; must equal nx, ny */
+    int px;               /* g_px: detector bins */
+    int pa;               /* g_pa: projection angles */
+    int ss;               /* g_ss: ordered subsets; pa % ss == 0 */
+    int zp;               /* g_zp: FBP zero-padding length, power of two >= px */
+    int mode;             /* g_mode: 3 (ASIRT) selects wt2 as the AART factor */
+    int correction_mode;  /* correctionMode: 0 off, 1-4 (reconstShare.cl:53-68) */
+    float wt1, wt2;       /* g_wt1, g_wt2: AART / ASIRT relaxation */
+    float cs_lambda;      /* CSlambda: the LAMBDA_FISTA build constant */
+} ctr_geometry;
+
+/* One slice of an uneven batched stack: square n x n image, n detector bins,
+ * pa angles shared by the whole batch (the ctr_set_geometry angle table). */
+typedef struct ctr_slice {
+    int n;                /* side: nx = ny = px = n */
+    float *img;           /* device [n][n], initial image in, result out */
+    const float *prj;     /* device [pa][n] sinogram */
+} ctr_slice;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` with its own non-blocking stream.
+ * Replaces OCL_platform_device (share source/OCL_platform_device_class.cpp:55-208):
+ * one device + one in-order queue. */
+ctr_status ctr_create(int device, ctr_ctx **out);
+ctr_status ctr_destroy(ctr_ctx *ctx);
+const char *ctr_last_error(const ctr_ctx *ctx);
+const char *ctr_status_string(ctr_status status);
+
+/* Uses `hip_stream` (a hipStream_t) instead of the context's own stream;
+ * NULL restores the context stream. */
+ctr_status ctr_set_stream(ctr_ctx *ctx, void *hip_stream);
+ctr_status ctr_synchronize(ctr_ctx *ctx);
+
+/* 1: synchronise after every kernel launch, as GPU.cpp does with finish()
+ * after each enqueue (GPU.cpp:216-227).  0 (default): stream-ordered only. */
+ctr_status ctr_set_sync_each_kernel(ctr_ctx *ctx, int enable);
+
+/* Sets the geometry, the host angle table (pa float degrees, the angle log of
+ * fileIO.cpp:11-24) and the subset order (ss ints; NULL = the reference order
+ * of GPU.cpp:564-582).  Replaces *_programBuild (GPU.cpp:513-559, 922-995,
+ * 1311-1354, 2023-2077) and the angle-buffer upload (GPU.cpp:619-624). */
+ctr_status ctr_set_geometry(ctr_ctx *ctx, const ctr_geometry *geom,
+                            const float *angles_deg, const int *sub);
+
+/* Writes the reference subset order for `ss` subsets (GPU.cpp:564-582). */
+ctr_status ctr_subset_order(int ss, int *sub);
+
+/* ---- execution drivers (drop-ins for GPU.cpp) --------------------------- */
+
+/* OSEM_execution, GPU.cpp:125-240 (declared CT_reconstruction.hpp:77). */
+ctr_status ctr_osem_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
+                              int dN, int it, int prj_correction);
+
+/* FISTA_OSEM_execution, GPU.cpp:243-405 (CT_reconstruction.hpp:78-81). */
+ctr_status ctr_fista_osem_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
+                                    int dN, int it, int prj_correction);
+
+/* AART_execution, GPU.cpp:1088-1216 (CT_reconstruction.hpp:82). */
+ctr_status ctr_aart_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
+                              int dN, int it);
+
+/* FBP_execution, GPU.cpp:665-819 (log2(zp) stages, GPU.cpp:855). */
+ctr_status ctr_fbp_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
+                             int dN, int prj_correction);
+
+/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is
+ * 30 in the reference. */
+ctr_status ctr_power_iteration(ctr_ctx *ctx, float *L2norm, int trials);
+
+/* FISTA_execution, GPU.cpp:1755-1924 (CT_reconstruction.hpp:83), with the
+ * reference's kernel-argument bindings (the returned image is the first
+ * ISTA3D step for any it >= 1). */
+ctr_status ctr_fista_execution(ctr_ctx *ctx, const float *L2norm, float *reconst_img,
+                               float *prj_img, int dN, int it, int prj_correction);
+
+/* ---- single operators (device buffers, dN layers) ----------------------- */
+
+/* OSEM1, OSEM.cl:38-68: writes rows th = sub + j*ss of rprj [dN][pa][px]. */
+ctr_status ctr_op_osem1(ctr_ctx *ctx, const float *img, const float *prj, float *rprj,
+                        int dN, int sub);
+/* OSEM2, OSEM.cl:71-105: img_dst = img * BP(rprj) * ss / pa. */
+ctr_status ctr_op_osem2(ctr_ctx *ctx, const float *img, const float *rprj, float *img_dst,
+                        int dN, int sub);
+/* AART1, AART.cl:38-69: rows of dprj = y - A x. */
+ctr_status ctr_op_aart1(ctr_ctx *ctx, const float *img, const float *prj, float *dprj,
+                        int dN, int sub);
+/* AART2, AART.cl:73-106. */
+ctr_status ctr_op_aart2(ctr_ctx *ctx, const float *img, const float *dprj, float *img_dst,
+                        int dN, int sub, float alpha);
+/* backProjection, reconstShare.cl:245-275. */
+ctr_status ctr_op_backprojection(ctr_ctx *ctx, const float *prj, float *img_dst,
+                                 int dN, int sub);
+/* sinogramCorrection, reconstShare.cl:35-70 (a = b = 0.5 as the host passes). */
+ctr_status ctr_op_sinogram_correction(ctr_ctx *ctx, const float *src, float *dst,
+                                      int dN, int mode);
+/* FBP.cl:34-160 chain (spinFactor .. outputImage): ramp-filtered rows. */
+ctr_status ctr_op_fbp_filter(ctr_ctx *ctx, const float *prj, float *fprj, int dN);
+/* backProjectionFBP, FBP.cl:164-188. */
+ctr_status ctr_op_fbp_backprojection(ctr_ctx *ctx, const float *fprj, float *img, int dN);
+/* powerIter1, FISTA.cl:50-75: img and prjc hold ss layers; rows j < pa/ss. */
+ctr_status ctr_op_power_iter1(ctr_ctx *ctx, const float *img, float *prjc);
+/* powerIter2, FISTA.cl:79-110. */
+ctr_status ctr_op_power_iter2(ctr_ctx *ctx, const float *prjc, float *cnd);
+/* FISTAbackProjection, FISTA.cl:211-246. */
+ctr_status ctr_op_fista_backprojection(ctr_ctx *ctx, const float *img, const float *dprj,
+                                       const float *L2norm, float *img_dst, int dN, int sub,
+                                       float alpha);
+/* ISTA3D, FISTA.cl:409-493 (lambda = cs_lambda / L2norm[sub] * 0.5). */
+ctr_status ctr_op_ista3d(ctr_ctx *ctx, const float *v, float *x_new, int dN, int sub,
+                         const float *L2norm);
+/* FISTA3D, FISTA.cl:497-594.  Any output may be NULL to skip it; when
+ * x_new == b_new the beta value is what remains (reference write order). */
+ctr_status ctr_op_fista3d(ctr_ctx *ctx, const float *x, const float *v, const float *b,
+                          float *w_new, float *x_new, float *b_new, int dN, int sub,
+                          const float *L2norm);
+
+/* ---- batched uneven stacks (no reference counterpart: the reference runs one
+ *      layer batch of equal size per OSEM_execution call) ------------------- */
+
+/* Runs OS-EM (OSEM_execution semantics, no correction) on `count` slices of
+ * possibly different sides in one stream-ordered schedule: per subset, one
+ * launch per kernel covers every slice.  use_graph != 0 captures one
+ * iteration in a hipGraph and replays it.  Workspace and graphs are cached
+ * in the context for repeated calls with the same slice list. */
+ctr_status ctr_osem_batched(ctr_ctx *ctx, const ctr_slice *slices, int count, int it,
+                            int use_graph);
+
+/* Times the kernel variants for slice side n on the context's device and
+ * keeps the fastest (waves per projection block, rows per backprojection
+ * thread).  Returns the choice in *fp_waves / *bp_rows (may be NULL). */
+ctr_status ctr_autotune(ctr_ctx *ctx, int n, int *fp_waves, int *bp_rows);
+
+/* Average duration (ms) of the last timed forward-projection and
+ * backprojection launches recorded with ctr_set_kernel_timing(ctx, 1). */
+ctr_status ctr_set_kernel_timing(ctr_ctx *ctx, int enable);
+ctr_status ctr_kernel_timing(ctr_ctx *ctx, double *fp_ms, double *bp_ms, int *fp_launches,
+                             int *bp_launches);
+
+/* ---- differentiable alignment model (no reference counterpart) ---------- *
+ * Forward model: the OSEM1 projector (OSEM.cl:56-62) over all pa angles with
+ * the detector coordinate of projection a moved by shifts[a] + center:
+ *   xi = (X - px/2) - shifts[a] - center.  shifts may be NULL (all zero). */
+ctr_status ctr_align_forward(ctr_ctx *ctx, const float *img, const float *shifts,
+                             float center, float *prj, int dN);
+/* Exact adjoint of ctr_align_forward with respect to the image. */
+ctr_status ctr_align_adjoint(ctr_ctx *ctx, const float *prj, const float *shifts,
+                             float center, float *img, int dN);
+/* gshift[a] = d/d shifts[a] of sum(gprj * forward(img)); summed over the dN
+ * layers (shifts are shared).  d/d center = sum_a gshift[a]. */
+ctr_status ctr_align_grad_shift(ctr_ctx *ctx, const float *img, const float *gprj,
+                                const float *shifts, float center, float *gshift, int dN);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
