@@ -115,7 +115,7 @@ def main():
     tune = None
     if not args.no_tune:
         tune = ctx.autotune(n)
-        log("autotune (fp waves/block, bp rows/thread):", tune)
+        log("autotune (fp rays/wave, fp waves/block, bp rows/thread):", tune)
 
     # first call: one iteration from the uniform start -> parity sample of slice 0
     ctx.osem_batched(imgs, sinos, 1, use_graph=True)

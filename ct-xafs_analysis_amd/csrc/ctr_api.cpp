@@ -59,7 +59,7 @@ struct SliceSet {
     std::vector<int> hmap[6];
     DevBuf dsl, dmap[6];
     int nb[6] = {0, 0, 0, 0, 0, 0};
-    int fp_wpb = 4, bp_rows = 4, na = 1;
+    int fp_wpb = 4, fp_rays = 64, bp_rows = 4, na = 1;
 };
 
 struct Layout {
@@ -78,6 +78,10 @@ Layout make_layout(int nx, int ny, int px)
 
 size_t pimg_floats(const Layout &l) { return (size_t)(l.ny + 2 * PAD) * l.ipitch; }
 size_t pimgT_floats(const Layout &l) { return (size_t)(l.nx + 2 * PAD) * l.tpitch; }
+
+struct Tune {
+    int fp_wpb = 4, fp_rays = 64, bp_rows = 4;
+};
 
 struct BatchPlan {
     std::vector<ctr_slice> key;
@@ -109,8 +113,8 @@ struct ctr_ctx {
     // stack workspace
     DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull;
     SliceSet set_a, set_b;
-    // autotune table: n -> (fp waves per block, bp rows per thread)
-    std::map<int, std::pair<int, int>> tune;
+    // autotune table: n -> schedule
+    std::map<int, Tune> tune;
     std::unique_ptr<BatchPlan> plan;
     // kernel timing
     bool timing = false;
@@ -171,21 +175,23 @@ ctr_status post(ctr_ctx *c, const char *what)
         if (s_ != CTR_OK) return s_;             \
     } while (0)
 
-std::pair<int, int> tuned(ctr_ctx *c, int n)
+Tune tuned(ctr_ctx *c, int n)
 {
     auto it = c->tune.find(n);
     if (it != c->tune.end()) return it->second;
-    return {4, 4};
+    return Tune{};
 }
 
 // Fills `S` with one descriptor per slice and uploads descriptors and maps.
 ctr_status upload_set(ctr_ctx *c, SliceSet &S)
 {
+    // the previous upload from these host vectors must have completed
+    HIPTRY(c, hipStreamSynchronize(c->stream));
     const int n = (int)S.h.size();
     for (int k = 0; k < 6; ++k) S.hmap[k].clear();
     for (int s = 0; s < n; ++s) {
         SliceDev &d = S.h[s];
-        const int cnt[6] = {fp_blocks(d.px, S.na, S.fp_wpb), bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny),
+        const int cnt[6] = {fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays), bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny),
                             pad_blocks(d.ny, d.ipitch), ew_blocks(d.nx, d.ny), 0};
         for (int k = 0; k < 6; ++k) {
             d.blk[k] = (int)S.hmap[k].size();
@@ -234,9 +240,10 @@ ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
     const ctr_geometry &g = c->g;
     const Layout l = make_layout(g.nx, g.ny, g.px);
     const int na = g.pa / g.ss;
-    const auto t = tuned(c, std::max(g.nx, g.ny));
-    S.fp_wpb = t.first;
-    S.bp_rows = t.second;
+    const Tune t = tuned(c, std::max(g.nx, g.ny));
+    S.fp_wpb = t.fp_wpb;
+    S.fp_rays = t.fp_rays;
+    S.bp_rows = t.bp_rows;
     S.na = na;
     S.h.assign(nz, SliceDev{});
     for (int z = 0; z < nz; ++z) {
@@ -320,7 +327,7 @@ ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer,
 {
     FpParams p{sub, c->g.ss, c->g.pa / c->g.ss, sub_from_layer, geo_power};
     time_begin(c, 0);
-    launch_fp(var, S.fp_wpb, launch_of(c, S, G_FP), p, c->stream);
+    launch_fp(var, S.fp_wpb, S.fp_rays, launch_of(c, S, G_FP), p, c->stream);
     POST(c, "forward projection");
     time_end(c, 0);
     return CTR_OK;
@@ -998,9 +1005,10 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
         launch_fill(P.rows.f(), fr, 0.0f, c->stream);
         POST(c, "fill rows");
         const int n0 = slices[order[0]].n;
-        const auto t = tuned(c, n0);
-        P.set.fp_wpb = t.first;
-        P.set.bp_rows = t.second;
+        const Tune t = tuned(c, n0);
+        P.set.fp_wpb = t.fp_wpb;
+        P.set.fp_rays = t.fp_rays;
+        P.set.bp_rows = t.bp_rows;
         P.set.na = na;
         P.set.h.assign(count, SliceDev{});
         size_t oi = 0, ot = 0, orr = 0;
@@ -1073,13 +1081,32 @@ extern "C" ctr_status ctr_kernel_timing(ctr_ctx *c, double *fp_ms, double *bp_ms
     return CTR_OK;
 }
 
-extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_waves, int *bp_rows)
+extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, int fp_rays, int fp_waves, int bp_rows)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    const bool ok = (fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
+                    (fp_waves == 2 || fp_waves == 4) &&
+                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8) && n >= 2;
+    if (!ok)
+        return fail(c, CTR_INVALID_ARGUMENT, "schedule (n=%d, rays=%d, waves=%d, rows=%d) not supported", n, fp_rays,
+                    fp_waves, bp_rows);
+    Tune t;
+    t.fp_rays = fp_rays;
+    t.fp_wpb = fp_waves;
+    t.bp_rows = bp_rows;
+    c->tune[n] = t;
+    c->plan.reset();
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_waves, int *bp_rows)
 {
     TRY(need_geom(c));
     if (n < 2 || n > 16384) return fail(c, CTR_INVALID_ARGUMENT, "side %d out of range", n);
     HIPTRY(c, hipSetDevice(c->device));
     const int na = c->g.pa / c->g.ss;
-    const int reps = 3;
+    const int reps = 5;
+    const int copies = 8;   // several descriptors on one slice: launches of batch size
     // a synthetic slice: disk image, ones sinogram
     const Layout l = make_layout(n, n, n);
     DevBuf pimg, pimgT, rows, prj;
@@ -1094,11 +1121,12 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_waves, int *bp_row
     POST(c, "autotune fill");
     SliceSet S;
     S.na = na;
-    S.h.assign(1, SliceDev{});
-    SliceDev &d = S.h[0];
-    d.nx = d.ny = d.px = n;
-    d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
-    d.img = pimg.f(); d.imgT = pimgT.f(); d.rows = rows.f(); d.prj = prj.f();
+    S.h.assign(copies, SliceDev{});
+    for (SliceDev &d : S.h) {
+        d.nx = d.ny = d.px = n;
+        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+        d.img = pimg.f(); d.imgT = pimgT.f(); d.rows = rows.f(); d.prj = prj.f();
+    }
     hipEvent_t e0, e1;
     HIPTRY(c, hipEventCreate(&e0));
     HIPTRY(c, hipEventCreate(&e1));
@@ -1112,35 +1140,40 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_waves, int *bp_row
         (void)hipEventElapsedTime(&ms, e0, e1);
         return ms / reps;
     };
-    int best_w = 4, best_r = 4;
+    Tune best_t;
     double best = 1e30;
-    const int wopts[3] = {2, 4, 8};
-    for (int w : wopts) {
-        S.fp_wpb = w;
-        S.bp_rows = 4;
-        TRY(upload_set(c, S));
-        FpParams p{0, c->g.ss, na, 0, 0};
-        const double ms = timeit([&] { launch_fp(FP_RATIO, w, launch_of(c, S, G_FP), p, c->stream); });
-        if (ms < best) { best = ms; best_w = w; }
-    }
+    const int ray_opts[4] = {64, 32, 16, 8};
+    const int wpb_opts[2] = {2, 4};
+    for (int rays : ray_opts)
+        for (int w : wpb_opts) {
+            S.fp_wpb = w;
+            S.fp_rays = rays;
+            S.bp_rows = 4;
+            TRY(upload_set(c, S));
+            FpParams p{0, c->g.ss, na, 0, 0};
+            const double ms = timeit([&] { launch_fp(FP_RATIO, w, rays, launch_of(c, S, G_FP), p, c->stream); });
+            if (ms < best) { best = ms; best_t.fp_wpb = w; best_t.fp_rays = rays; }
+        }
     best = 1e30;
     const int ropts[3] = {2, 4, 8};
     for (int r : ropts) {
-        S.fp_wpb = best_w;
+        S.fp_wpb = best_t.fp_wpb;
+        S.fp_rays = best_t.fp_rays;
         S.bp_rows = r;
         TRY(upload_set(c, S));
         BpParams p{0, c->g.ss, na, c->g.pa, 0, 0.0f, nullptr, 0};
         const double ms = timeit([&] { launch_bp(BP_OSEM2, r, launch_of(c, S, G_BP), p, c->stream); });
-        if (ms < best) { best = ms; best_r = r; }
+        if (ms < best) { best = ms; best_t.bp_rows = r; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPTRY(c, hipGetLastError());
     HIPTRY(c, hipStreamSynchronize(c->stream));
-    c->tune[n] = {best_w, best_r};
+    c->tune[n] = best_t;
     c->plan.reset();   // plans pick up the new choice on rebuild
-    if (fp_waves) *fp_waves = best_w;
-    if (bp_rows) *bp_rows = best_r;
+    if (fp_rays) *fp_rays = best_t.fp_rays;
+    if (fp_waves) *fp_waves = best_t.fp_wpb;
+    if (bp_rows) *bp_rows = best_t.bp_rows;
     return CTR_OK;
 }
 

@@ -62,7 +62,7 @@ struct BpParams {
 };
 
 // ---- launchers (ctr_kernels.hip) ----
-void launch_fp(int var, int wpb, const Launch &L, const FpParams &p, hipStream_t stream);
+void launch_fp(int var, int wpb, int rays, const Launch &L, const FpParams &p, hipStream_t stream);
 void launch_bp(int var, int rows, const Launch &L, const BpParams &p, hipStream_t stream);
 void launch_transpose(const Launch &L, hipStream_t stream);
 void launch_pad_in(const Launch &L, hipStream_t stream);
@@ -95,7 +95,7 @@ void launch_align_grad(const float *pimg, const float *pimgT, int ipitch, int tp
                        int nz, int nx, int ny, int pa, int px, hipStream_t stream);
 
 // blocks per slice for each grid kind
-int fp_blocks(int px, int na, int wpb);
+int fp_blocks(int px, int na, int wpb, int rays);
 int bp_blocks(int nx, int ny, int rows);
 int tr_blocks(int nx, int ny);
 int pad_blocks(int ny, int ipitch);

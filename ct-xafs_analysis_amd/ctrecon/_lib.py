@@ -71,7 +71,8 @@ _SIGS = {
     "ctr_op_ista3d": [_P, _P, _P, _I, _I, _P],
     "ctr_op_fista3d": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P],
     "ctr_osem_batched": [_P, C.POINTER(Slice), _I, _I, _I],
-    "ctr_autotune": [_P, _I, C.POINTER(_I), C.POINTER(_I)],
+    "ctr_autotune": [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
+    "ctr_set_schedule": [_P, _I, _I, _I, _I],
     "ctr_set_kernel_timing": [_P, _I],
     "ctr_kernel_timing": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)],
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
@@ -223,9 +224,13 @@ class Context:
         self._chk(self.lib.ctr_osem_batched(self.h, arr, n, it, int(use_graph)), "ctr_osem_batched")
 
     def autotune(self, n):
-        w, r = C.c_int(), C.c_int()
-        self._chk(self.lib.ctr_autotune(self.h, n, C.byref(w), C.byref(r)), "ctr_autotune")
-        return w.value, r.value
+        """Returns (rays per projection wave, waves per projection block, rows per backprojection thread)."""
+        rays, w, r = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.lib.ctr_autotune(self.h, n, C.byref(rays), C.byref(w), C.byref(r)), "ctr_autotune")
+        return rays.value, w.value, r.value
+
+    def set_schedule(self, n, fp_rays, fp_waves, bp_rows):
+        self._chk(self.lib.ctr_set_schedule(self.h, n, fp_rays, fp_waves, bp_rows), "ctr_set_schedule")
 
     def set_kernel_timing(self, on):
         self._chk(self.lib.ctr_set_kernel_timing(self.h, int(bool(on))), "ctr_set_kernel_timing")
