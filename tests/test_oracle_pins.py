@@ -1,0 +1,297 @@
+"""Pins of the CPU oracle (oracle/ct_oracle*.c) without running the reference.
+
+The reference OpenCL kernels cannot execute here (no OpenCL CPU device), and the
+reference ships no tests or golden data, so each oracle function is pinned by
+something independent of its own formula:
+
+  subset order            literal lists (GPU.cpp:564-582 worked by hand)
+  forward projector       theta=0 closed form (column sums), translation
+                          equivariance, 90-degree rotation symmetry, FP32 vs FP64
+  alignment adjoint       dot-product test against the independently written
+                          scatter adjoint (FP64)
+  shift gradient          FP64 central finite differences of the forward model
+  nearest backprojection  theta=0 identity, normalisation fixed point of OSEM2
+  NaN / clamp rules       crafted inputs (OSEM.cl:65-67, 101; AART.cl:69;
+                          reconstShare.cl:268; FISTA.cl:488-489)
+  FBP filter              brute-force O(ZP^2) inverse DFT of the ramp in numpy
+  FBP amplitude           uniform disk reconstructs to mu/2 (FBP.cl:186)
+  TV prox                 brute-force minimisation of 1/2(x-v)^2 + lam sum|x-n_i|
+  power iteration         dense eigen-decomposition of the 16^2 system
+  driver quirks           FISTA_execution returns x1; AART init replaces the
+                          image; FISTA_OSEM discards its first OS-EM update
+  phantom                 analytic line integral of a centred disk
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from _util import relerr
+
+F32, F64 = np.float32, np.float64
+
+
+def test_subset_order_literals():
+    assert O.subset_order(16).tolist() == [0, 8, 4, 12, 2, 10, 6, 14, 1, 9, 5, 13, 3, 11, 7, 15]
+    assert O.subset_order(8).tolist() == [0, 4, 2, 6, 1, 5, 3, 7]
+    assert O.subset_order(10).tolist() == list(range(10))
+    assert O.subset_order(1).tolist() == [0]
+    assert O.subset_order(6).tolist() == list(range(6))
+
+
+def _proj(g, ang, img, dtype):
+    """A x through AART1 with y = 0 (residual = -A x)."""
+    z = np.zeros((img.shape[0], g.pa, g.px), dtype=dtype)
+    return -O.aart1(g, ang, img.astype(dtype), z, 0, dtype=dtype)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_projector_theta0_column_sums(dtype):
+    n = 32
+    g = O.geom(n, n, 1, ss=1)
+    rng = np.random.default_rng(0)
+    img = rng.random((1, n, n))
+    p = _proj(g, np.array([0.0], np.float32), img, dtype)
+    # at theta = 0 with P = N = D the samples hit texel centres: p[X] = sum_y img[y][X]
+    assert np.allclose(p[0, 0], img[0].sum(axis=0), rtol=1e-6, atol=1e-5)
+
+
+def test_projector_translation_and_rotation():
+    n = 32
+    g = O.geom(n, n, 2, ss=1)
+    rng = np.random.default_rng(1)
+    img = np.zeros((1, n, n))
+    img[0, 8:24, 8:24] = rng.random((16, 16))
+    ang = np.array([0.0, 90.0], np.float32)
+    p = _proj(g, ang, img, F64)
+    # a one-pixel shift in x moves the theta=0 projection one bin
+    p_s = _proj(g, ang, np.roll(img, 1, axis=2), F64)
+    assert np.allclose(p_s[0, 0, 1:], p[0, 0, :-1], atol=1e-9)
+    # theta = 90 projects along x onto the reversed y axis: s = -y
+    rows = img[0].sum(axis=1)
+    assert np.allclose(p[0, 1, 1:], rows[::-1][:-1], atol=1e-6)
+
+
+def test_projector_fp32_vs_fp64_tie_free():
+    n, pa = 48, 30
+    g = O.geom(n, n, pa, ss=1)
+    ang = (np.arange(pa) * 6.0 + 0.731).astype(np.float32)
+    img = np.random.default_rng(2).random((1, n, n))
+    assert relerr(_proj(g, ang, img, F32), _proj(g, ang, img, F64)) < 2e-6
+
+
+def test_align_adjoint_dot_product_fp64():
+    n, pa = 24, 17
+    g = O.geom(n, n, pa, ss=1)
+    rng = np.random.default_rng(3)
+    ang = rng.uniform(0, 180, pa).astype(np.float32)
+    sh = rng.normal(0, 1.5, pa)
+    for trial in range(3):
+        x = rng.normal(size=(1, n, n))
+        y = rng.normal(size=(1, pa, n))
+        ax = O.align_forward(g, ang, x, sh, 0.7)
+        aty = O.align_adjoint(g, ang, y, sh, 0.7)
+        lhs, rhs = np.vdot(ax, y), np.vdot(x, aty)
+        assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(ax) * np.linalg.norm(y)
+
+
+def test_align_forward_equals_osem_projector():
+    n, pa = 32, 12
+    g = O.geom(n, n, pa, ss=1)
+    ang = (np.arange(pa) * 15.0 + 0.3).astype(np.float32)
+    img = np.random.default_rng(4).random((1, n, n))
+    assert relerr(O.align_forward(g, ang, img, None, 0.0), _proj(g, ang, img, F64)) < 1e-12
+
+
+def test_align_shift_gradient_fd_fp64():
+    n, pa = 24, 9
+    g = O.geom(n, n, pa, ss=1)
+    rng = np.random.default_rng(5)
+    ang = rng.uniform(0, 180, pa).astype(np.float32)
+    from ctrecon import phantom as ph
+    img = ph.rasterize(ph.random_ellipses(4, rng), n, supersample=2)[None]
+    sh = rng.normal(0, 1.0, pa)
+    gp = rng.normal(size=(1, pa, n))
+    grad = O.align_grad(g, ang, img, gp, sh, 0.4)
+    h = 1e-6
+    fd = np.zeros(pa)
+    for a in range(pa):
+        sp, sm = sh.copy(), sh.copy()
+        sp[a] += h
+        sm[a] -= h
+        fd[a] = (np.vdot(gp, O.align_forward(g, ang, img, sp, 0.4))
+                 - np.vdot(gp, O.align_forward(g, ang, img, sm, 0.4))) / (2 * h)
+    assert relerr(grad, fd) < 1e-4
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_backprojection_theta0_identity(dtype):
+    n = 16
+    g = O.geom(n, n, 1, ss=1)
+    rows = np.random.default_rng(6).random((1, 1, n))
+    img = np.ones((1, n, n))
+    # OSEM2 at a single angle theta = 0: x' = x * r[bin(X)] * SS/A with bin(X) = X
+    out = O.osem2(g, np.array([0.0], np.float32), img, rows, 0, dtype=dtype)
+    assert np.allclose(out[0], np.broadcast_to(rows[0, 0], (n, n)), rtol=1e-6)
+
+
+def test_osem2_normalisation_fixed_point():
+    """With all ratios 1, OSEM2 multiplies x by (#in-range bins) * SS / A, which is
+    exactly 1 for pixels whose every subset bin falls on the detector."""
+    n, pa, ss = 32, 60, 6
+    g = O.geom(n, n, pa, ss=ss)
+    ang = (np.arange(pa) * 3.0).astype(np.float32)
+    x = np.random.default_rng(7).uniform(0.5, 1.5, (1, n, n))
+    out = O.osem2(g, ang, x, np.ones((1, pa, n)), 2, dtype=F64)
+    yy, xx = np.mgrid[0:n, 0:n]
+    inside = (xx - n / 2) ** 2 + (yy - n / 2) ** 2 < (n / 2 - 1.5) ** 2
+    assert np.allclose(out[0][inside], x[0][inside], rtol=1e-12)
+
+
+def test_nan_and_clamp_rules():
+    n, pa = 8, 4
+    g = O.geom(n, n, pa, ss=1)
+    ang = np.array([0, 45, 90, 135], np.float32)
+    img = np.zeros((1, n, n), np.float32)          # projection p = 0 < 1e-4
+    prj = np.ones((1, pa, n), np.float32)
+    prj[0, 0, 0] = np.nan
+    prj[0, 1, 1] = -5.0
+    r = O.osem1(g, ang, img, prj, 0)
+    assert r[0, 0, 0] == np.float32(1e-6)            # NaN -> 1e-6
+    assert r[0, 1, 1] == np.float32(1e-6)            # max(1e-6, y)
+    assert r[0, 2, 2] == 1.0                          # p < 1e-4: ratio = y
+    d = O.aart1(g, ang, img, prj, 0)
+    assert d[0, 0, 0] == 0.0                          # NaN y -> 0
+    g1 = O.geom(n, n, 1, ss=1)
+    bp = O.backprojection(g1, np.array([0.0], np.float32), prj[:, :1], 0)
+    assert np.isfinite(bp).all() and (bp[0, :, 0] == 0.0).all()     # NaN bin -> 0
+    assert np.allclose(bp[0, :, 1], 1.0 / n)                           # SS / A / D
+    # OSEM2: NaN product -> 1e-6
+    x = np.full((1, n, n), np.nan, np.float32)
+    out = O.osem2(g1, ang[:1], x, np.ones((1, 1, n), np.float32), 0)
+    assert (out == np.float32(1e-6)).all()
+    # ISTA3D clamps to >= 1e-6
+    L = np.ones(1, np.float32)
+    v = np.full((1, n, n), -3.0, np.float32)
+    assert (O.ista3d(g, v, L, 0, 1e-3) == np.float32(1e-6)).all()
+
+
+@pytest.mark.parametrize("px,zp", [(16, 32), (24, 64), (64, 128)])
+def test_fbp_filter_closed_form(px, zp):
+    """Filtered row = circular convolution with h = IDFT of H[k] = (pi/ZP) min(k, ZP-k),
+    the row placed at index (X - P/2) mod ZP (FBP.cl:44-57, 125-160)."""
+    g = O.geom(px, px, 3, ss=1, zp=zp)
+    rng = np.random.default_rng(8)
+    prj = rng.normal(size=(1, 3, px))
+    k = np.arange(zp)
+    H = (np.pi / zp) * np.minimum(k, zp - k)
+    nn = np.arange(zp)
+    h = (np.exp(2j * np.pi * np.outer(nn, k) / zp) @ H).real / zp      # brute-force IDFT
+    idx = (np.arange(px) - px // 2) % zp
+    exp = np.zeros_like(prj)
+    for t in range(3):
+        row = np.zeros(zp)
+        row[idx] = prj[0, t]
+        conv = np.array([np.sum(row * h[(m - nn) % zp]) for m in range(zp)])
+        exp[0, t] = conv[idx]
+    assert relerr(O.fbp_filter(g, prj, dtype=F64), exp) < 1e-12
+    assert relerr(O.fbp_filter(g, prj.astype(F32)), exp) < 1e-5
+
+
+def test_fbp_disk_amplitude_is_half_mu():
+    n, pa, mu, rad = 128, 360, 0.5, 40.0
+    g = O.geom(n, n, pa, ss=1, zp=256)
+    ang = (np.arange(pa) * 0.5).astype(np.float32)
+    s = np.arange(n) - n / 2
+    row = np.where(np.abs(s) < rad, 2 * mu * np.sqrt(np.maximum(rad ** 2 - s ** 2, 0)), 0.0)
+    prj = np.repeat(row[None, None], pa, axis=1)
+    img = O.fbp_execution(g, ang, prj, dtype=F64)[0]
+    yy, xx = np.mgrid[0:n, 0:n]
+    r = np.hypot(xx - n / 2, yy - n / 2)
+    centre = img[r < rad - 8].mean()
+    assert abs(centre - mu / 2) < 0.03 * mu, centre
+    # outside the disk only the small negative offset of a band-limited ramp remains
+    assert abs(img[(r > rad + 8) & (r < n / 2 - 4)].mean()) < 0.05 * mu
+
+
+def test_tv_prox_is_the_l1_prox():
+    """ISTA3D (nz = 1: both z-neighbours are the pixel itself) returns
+    argmin_x 1/2 (x - v)^2 + lam * sum_i |x - n_i| over its six neighbours."""
+    rng = np.random.default_rng(9)
+    n = 6
+    g = O.geom(n, n, 1, ss=1)
+    v = rng.uniform(0.2, 1.0, (1, n, n))
+    L = np.array([0.8])
+    lam_f = 0.05
+    lam = lam_f / L[0] * 0.5
+    out = O.ista3d(g, v, L, 0, lam_f, dtype=F64)
+    grid = np.linspace(-0.5, 1.6, 210001)
+    for (yy, xx) in [(2, 3), (1, 1), (4, 2), (0, 0), (5, 5)]:
+        nb = [v[0, yy, xx + 1] if xx + 1 < n else 0.0, v[0, yy, xx - 1] if xx >= 1 else 0.0,
+              v[0, yy + 1, xx] if yy + 1 < n else 0.0, v[0, yy - 1, xx] if yy >= 1 else 0.0,
+              v[0, yy, xx], v[0, yy, xx]]
+        f = 0.5 * (grid - v[0, yy, xx]) ** 2 + lam * sum(np.abs(grid - t) for t in nb)
+        assert abs(grid[np.argmin(f)] - max(out[0, yy, xx], 1e-6)) < 2e-5
+
+
+def test_power_iteration_dense_eigenvalue():
+    n, pa, ss = 16, 8, 2
+    g = O.geom(n, n, pa, ss=ss)
+    ang = (np.arange(pa) * 22.5 + 0.4).astype(np.float32)
+    L = O.power_iteration(g, ang, 30, dtype=F64)
+    for s in range(ss):
+        M = np.zeros((n * n, n * n))
+        for i in range(n * n):
+            e = np.zeros((ss, n, n))
+            e[s].flat[i] = 1.0
+            M[:, i] = O.power_iter2(g, ang, O.power_iter1(g, ang, e, dtype=F64), dtype=F64)[s].ravel()
+        lam = np.max(np.abs(np.linalg.eigvals(M)))
+        assert abs(L[s] - lam ** 2) / lam ** 2 < 1e-3, (s, L[s], lam ** 2)
+
+
+def _small_case(seed=10, n=24, pa=12, ss=4):
+    from ctrecon import phantom as ph
+    img, sino, ang = ph.make_slice(n, pa, 5, seed)
+    return O.geom(n, n, pa, ss=ss), ang, sino[None]
+
+
+def test_fista_execution_returns_first_step():
+    g, ang, y = _small_case()
+    L = O.power_iteration(g, ang, 30)
+    x0 = np.ones((1, g.ny, g.nx), np.float32)
+    a = O.fista_execution(g, ang, L, x0, y, 1, 1.0, 1e-3)
+    b = O.fista_execution(g, ang, L, x0, y, 4, 1.0, 1e-3)
+    assert np.array_equal(a, b)
+    c = O.fista_execution(g, ang, L, x0, y, 0, 1.0, 1e-3)
+    assert np.array_equal(c, O.backprojection(g, ang, y, int(O.subset_order(g.ss)[0])))
+
+
+def test_aart_init_replaces_image():
+    g, ang, y = _small_case()
+    a = O.aart_execution(g, ang, np.ones((1, g.ny, g.nx)), y, 1, 1.0)
+    b = O.aart_execution(g, ang, np.full((1, g.ny, g.nx), 9.0), y, 1, 1.0)
+    assert np.array_equal(a, b)
+
+
+def test_fista_osem_first_step_discards_osem_update():
+    g, ang, y = _small_case()
+    x0 = np.random.default_rng(11).uniform(0.5, 1.0, (1, g.ny, g.nx)).astype(np.float32)
+    one = O.fista_osem_execution(g, ang, x0, y, 1, 1e-3, sub=np.array([0]) if False else None)
+    # a single subset step: the OS-EM update is overwritten by ISTA3D(x0) with L = 0.5
+    g1 = O.geom(g.nx, g.px, g.pa, ss=1)
+    ista = O.ista3d(g1, x0, np.array([0.5], np.float32), 0, 1e-3)
+    g_one = O.geom(g.nx, g.px, g.pa, ss=1)
+    out = O.fista_osem_execution(g_one, ang, x0, y, 1, 1e-3)
+    assert np.array_equal(out, ista)
+    assert one.shape == x0.shape
+
+
+def test_phantom_disk_line_integral():
+    from ctrecon import phantom as ph
+    n = 64
+    ell = ph.Ellipses(np.array([0.0]), np.array([0.0]), np.array([0.5]), np.array([0.5]), np.array([0.3]),
+                      np.array([0.8]))
+    ang = np.array([0.0, 33.0, 90.0], np.float32)
+    p = ph.line_integrals(ell, n, ang, n)
+    s = (np.arange(n) - n / 2) / (n / 2)
+    exp = np.where(np.abs(s) < 0.5, 2 * 0.8 * np.sqrt(np.maximum(0.25 - s ** 2, 0)), 0)
+    assert np.allclose(p, exp[None], atol=1e-7)   # r^2 = a^2 (cos^2 + sin^2) rounds at the rim
