@@ -80,7 +80,7 @@ size_t pimg_floats(const Layout &l) { return (size_t)(l.ny + 2 * PAD) * l.ipitch
 size_t pimgT_floats(const Layout &l) { return (size_t)(l.nx + 2 * PAD) * l.tpitch; }
 
 struct Tune {
-    int fp_wpb = 4, fp_rays = 64, bp_rows = 4;
+    int fp_wpb = 4, fp_rays = 0, bp_rows = 8;   // LDS-tiled projection, 8-row backprojection
 };
 
 struct BatchPlan {
@@ -1084,7 +1084,7 @@ extern "C" ctr_status ctr_kernel_timing(ctr_ctx *c, double *fp_ms, double *bp_ms
 extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, int fp_rays, int fp_waves, int bp_rows)
 {
     if (!c) return CTR_INVALID_ARGUMENT;
-    const bool ok = (fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
+    const bool ok = (fp_rays == 0 || fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
                     (fp_waves == 2 || fp_waves == 4) &&
                     (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8) && n >= 2;
     if (!ok)
@@ -1142,10 +1142,11 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
     };
     Tune best_t;
     double best = 1e30;
-    const int ray_opts[4] = {64, 32, 16, 8};
+    const int ray_opts[5] = {0, 64, 32, 16, 8};
     const int wpb_opts[2] = {2, 4};
     for (int rays : ray_opts)
         for (int w : wpb_opts) {
+            if (rays == 0 && w != 4) continue;   // the LDS-tiled kernel has a fixed block
             S.fp_wpb = w;
             S.fp_rays = rays;
             S.bp_rows = 4;

@@ -2,10 +2,12 @@
 //
 // Internal data layout in HBM (one "slice" = one reconstruction layer):
 //   pimg   zero-padded image, PAD texels of zeros on every side, row pitch
-//          ipitch floats, (ny + 2 PAD) rows.  The padding lets the bilinear
-//          gather read all four taps without bounds tests: coordinates are
-//          clamped into the padded frame, and any clamped sample lands on
-//          zeros, which is exactly CLK_ADDRESS_CLAMP's border value.
+//          ipitch floats, (ny + 2 PAD) rows.  The padding lets the projectors
+//          read without bounds tests: a direct bilinear gather clamps its
+//          coordinates into the padded frame, where any clamped sample lands
+//          on zeros (exactly CLK_ADDRESS_CLAMP's border value), and an LDS
+//          tile box (at most 97 x 94 texels) that meets the interior lies
+//          entirely inside the frame because PAD >= 97.
 //   pimgT  the same image transposed (pitch tpitch, nx + 2 PAD rows).  Rays
 //          steeper than 45 degrees read it so that the 64 lanes of a wave,
 //          which hold 64 neighbouring detector bins, touch consecutive
@@ -20,7 +22,7 @@
 
 namespace ctr {
 
-constexpr int PAD = 2;
+constexpr int PAD = 100;
 
 struct SliceDev {
     float *img;          // padded image (pitch ipitch, ny + 2 PAD rows)

@@ -173,14 +173,16 @@ ctr_status ctr_osem_batched(ctr_ctx *ctx, const ctr_slice *slices, int count, in
                             int use_graph);
 
 /* Times the kernel variants for slice side n on the context's device and
- * keeps the fastest for that side: rays per forward-projection wave (64, 32,
- * 16 or 8; the other 64/rays lanes of a ray take interleaved samples), waves
- * per forward-projection block and pixel rows per backprojection thread.
+ * keeps the fastest for that side: the forward-projection schedule (fp_rays
+ * 0 = LDS-tiled blocks of 64 rays; 64, 32, 16 or 8 = rays per wave of the
+ * direct-gather kernel, the other 64/rays lanes of a ray taking interleaved
+ * samples), waves per forward-projection block and pixel rows per
+ * backprojection thread.
  * Returns the choice through the pointers (each may be NULL). */
 ctr_status ctr_autotune(ctr_ctx *ctx, int n, int *fp_rays, int *fp_waves, int *bp_rows);
 
 /* Sets the schedule for slice side n explicitly (the values ctr_autotune
- * would choose): fp_rays in {64, 32, 16, 8}, fp_waves in {2, 4},
+ * would choose): fp_rays in {0, 64, 32, 16, 8}, fp_waves in {2, 4},
  * bp_rows in {1, 2, 4, 8}. */
 ctr_status ctr_set_schedule(ctr_ctx *ctx, int n, int fp_rays, int fp_waves, int bp_rows);
 
