@@ -30,10 +30,13 @@ sys.path.insert(0, os.path.join(ROOT, "ct-xafs_analysis_amd"))
 METRIC = "batched OS-EM slice-iters/s vs per-slice launch, 1024² 1200 proj, relerr≤1e-4"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per forward-projection launch at this workload (128 x 1024^2 slices),
-# measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the same
-# counters on a plain copy kernel (profiles/fp_kernel_1024_counters.md).
-FP_TRAFFIC_BYTES_PER_SLICE = None
+# Bytes moved past L2 per forward-projection launch, per slice of 1024^2 with
+# 120 angles (one subset of 1200): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# the 128-slice k_fp_lds<0> launch, scaled by the same counters on a plain
+# 256 MiB copy (FETCH x 1.99975, WRITE x 1.0), divided by 128
+# (profiles/k_fp_lds_k_bp_osem_1024sq_1200proj_128slices.md).
+FP_TRAFFIC_BYTES_PER_SLICE = 155394111.0
+FP_TRAFFIC_SHAPE = (1024, 1200, 10)
 
 
 def log(*a):
@@ -92,6 +95,8 @@ def main():
     ap.add_argument("--per-slice", type=int, default=4, help="slices timed in the per-slice-launch baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (reference size for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box here has one GPU.",
@@ -130,6 +135,14 @@ def main():
     ctx.set_kernel_timing(False)
     log(f"FP kernel {fp_ms:.3f} ms/launch, BP kernel {bp_ms:.3f} ms/launch ({nfp}, {nbp} launches)")
 
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"reference copy of {args.copy_mib} MiB done")
+        del src, dst
+
     for _ in range(args.warmup):
         ctx.osem_batched(imgs, sinos, 1, use_graph=True)
     ctx.synchronize()
@@ -163,15 +176,19 @@ def main():
     log(f"per-slice launch: {per_slice:.2f} slice-iters/s")
 
     na = pa // ss
+    fp_kernel = "k_fp_lds" if (tune is None or tune[0] == 0) else "k_fp"
     fp_flops = 14.0 * n * na * n * B            # 14 FLOP per bilinear sample, D = n samples per ray
+    fp_bytes = (4 * n * n + 8 * na * n) * B     # image once, measured rows in, ratio rows out
     fp_tflops = fp_flops / (fp_ms * 1e-3) / 1e12
     step_flops = (14.0 * n * pa * n + 2.0 * n * n * pa) * B
     hbm_bytes = (12.0 * ss * n * n + 12.0 * n * pa) * B
     roofline = {
         "bound": "valu", "achieved": round(fp_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": round(fp_tflops / PEAK_FP32_TFLOPS, 4),
-        "traffic": (FP_TRAFFIC_BYTES_PER_SLICE * B if FP_TRAFFIC_BYTES_PER_SLICE else None),
-        "kernel": "k_fp<FP_RATIO> (OSEM1: forward projection + ratio), one launch = one subset of every slice",
+        "traffic": (round(FP_TRAFFIC_BYTES_PER_SLICE * B) if (n, pa, ss) == FP_TRAFFIC_SHAPE and fp_kernel == "k_fp_lds"
+                    else None),
+        "algorithmic_bytes": fp_bytes,
+        "kernel": f"{fp_kernel}<FP_RATIO> (OSEM1: forward projection + ratio), one launch = one subset of every slice",
         "fp_ms_per_launch": round(fp_ms, 4), "bp_ms_per_launch": round(bp_ms, 4),
         "whole_step_tflops": round(step_flops * value / B / 1e12, 3),
         "whole_step_hbm_frac": round(hbm_bytes * value / B / (PEAK_HBM_GBS * 1e9), 5),
