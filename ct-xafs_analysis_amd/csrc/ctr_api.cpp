@@ -178,8 +178,12 @@ ctr_status post(ctr_ctx *c, const char *what)
 Tune tuned(ctr_ctx *c, int n)
 {
     auto it = c->tune.find(n);
-    if (it != c->tune.end()) return it->second;
-    return Tune{};
+    Tune t = it != c->tune.end() ? it->second : Tune{};
+    if (t.fp_rays == 0 && n > LDS_FP_MAX_SIDE) {
+        t.fp_rays = 64;
+        t.fp_wpb = 2;
+    }
+    return t;
 }
 
 // Fills `S` with one descriptor per slice and uploads descriptors and maps.
@@ -1086,7 +1090,8 @@ extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, int fp_rays, int fp_wa
     if (!c) return CTR_INVALID_ARGUMENT;
     const bool ok = (fp_rays == 0 || fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
                     (fp_waves == 2 || fp_waves == 4) &&
-                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8) && n >= 2;
+                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8) && n >= 2 &&
+                    !(fp_rays == 0 && n > LDS_FP_MAX_SIDE);
     if (!ok)
         return fail(c, CTR_INVALID_ARGUMENT, "schedule (n=%d, rays=%d, waves=%d, rows=%d) not supported", n, fp_rays,
                     fp_waves, bp_rows);
@@ -1146,7 +1151,7 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
     const int wpb_opts[2] = {2, 4};
     for (int rays : ray_opts)
         for (int w : wpb_opts) {
-            if (rays == 0 && w != 4) continue;   // the LDS-tiled kernel has a fixed block
+            if (rays == 0 && (w != 4 || n > LDS_FP_MAX_SIDE)) continue;   // the LDS-tiled kernel has a fixed block
             S.fp_wpb = w;
             S.fp_rays = rays;
             S.bp_rows = 4;
