@@ -295,3 +295,21 @@ def test_phantom_disk_line_integral():
     s = (np.arange(n) - n / 2) / (n / 2)
     exp = np.where(np.abs(s) < 0.5, 2 * 0.8 * np.sqrt(np.maximum(0.25 - s ** 2, 0)), 0)
     assert np.allclose(p, exp[None], atol=1e-7)   # r^2 = a^2 (cos^2 + sin^2) rounds at the rim
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_sinogram_correction_closed_forms(mode):
+    """reconstShare.cl:35-70 with a = b = 0.5: detector weight sqrt(1 - t^2)
+    (= cos asin t), angle weight cos(theta), both, or the mask
+    t^2 < (a + b cos theta)^2, with t = |2X - P| / P."""
+    px, pa = 20, 12
+    g = O.geom(16, px, pa)
+    ang = (np.arange(pa) * 15.0).astype(np.float32)
+    y = np.random.default_rng(11).uniform(0.5, 2.0, (2, pa, px))
+    got = O.sinogram_correction(g, ang, y, mode, dtype=F64)
+    t = np.abs(2.0 * np.arange(px) - px) / px
+    th = np.deg2rad(ang.astype(np.float64))[:, None]
+    wx, wt = np.sqrt(1.0 - t * t)[None, :], np.cos(th)
+    want = {0: y, 1: y * wx, 2: y * wt, 3: y * wx * wt,
+            4: np.where((t * t)[None, :] < (0.5 + 0.5 * wt) ** 2, y, 0.0)}[mode]
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
