@@ -1,10 +1,16 @@
 """Benchmark: batched OS-EM slice-iterations/s on one MI355X.
 
-Workload: a batch of B slices of 1024 x 1024, 1200 projections over [0, 180),
-1024 bins, 10 ordered subsets (the 1024^2 / 1200-projection case of the
-batched OS-EM stack).  A step is one OS-EM iteration (all 10 subsets) over the
-whole batch through ctr_osem_batched, with the sinograms and images already
-resident in HBM; value = B * steps / time.
+Default workload: a batch of B slices of 1024 x 1024, 1200 projections over
+[0, 180), 1024 bins, 10 ordered subsets (the 1024^2 / 1200-projection case of
+the batched OS-EM stack).  A step is one OS-EM iteration (all 10 subsets) over
+the whole batch through ctr_osem_batched, with the sinograms and images
+already resident in HBM; value = B * steps / time.
+
+--workload mixed runs the uneven CT-XAFS stack instead: E energies x S slices,
+each slice's side seeded from {256, 384, 512, 768, 1024} and shared by its
+energies, attenuation with a sigmoid absorption edge over the energies.
+value is then in 1024^2-equivalent slice-iterations/s,
+sum(n^2 / 1024^2) * steps / time.
 
 Printed beside it:
   per_slice_launch   the same kernels launched one slice at a time with a
@@ -16,7 +22,7 @@ Printed beside it:
                      algorithmic FLOPs per launch over its mean launch time,
                      measured with HIP events on the library stream
   cpu_baseline       the oracle's FP32 restatement (OpenMP, all host threads)
-                     timed on one slice-iteration of slice 0
+                     timed on one slice-iteration of a 1024^2 slice
 """
 import argparse
 import json
@@ -45,7 +51,6 @@ def log(*a):
 
 def make_batch(B, n, pa, seed, device):
     import numpy as np
-    import torch
 
     from ctrecon import phantom as ph
     ang = ph.uniform_angles(pa)
@@ -61,25 +66,26 @@ def make_batch(B, n, pa, seed, device):
     return ang, sinos
 
 
-def cpu_baseline(n, pa, ss, ang, sino0, gpu_slice0):
-    """FP32 compat oracle and FP64 oracle on one slice-iteration of slice 0."""
+def cpu_baseline(n, pa, ss, ang, sino0, gpu_slice0, n_time, sino_time):
+    """Oracle FP32 compat and FP64 on one slice-iteration of slice 0 (the
+    parity sample), and the FP32 oracle timed on one n_time^2 slice-iteration."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    g = O.geom(n, n, pa, ss=ss)
     x0 = np.ones((1, n, n), np.float32)
-    t0 = time.perf_counter()
+    g = O.geom(n, n, pa, ss=ss)
     c32 = O.osem_execution(g, ang, x0, sino0[None], 1)
-    t32 = time.perf_counter() - t0
-    t0 = time.perf_counter()
     c64 = O.osem_execution(g, ang, x0, sino0[None], 1, dtype=np.float64)
-    t64 = time.perf_counter() - t0
+    gt = O.geom(n_time, n_time, pa, ss=ss)
+    xt = np.ones((1, n_time, n_time), np.float32)
+    t0 = time.perf_counter()
+    O.osem_execution(gt, ang, xt, sino_time[None], 1)
+    t32 = time.perf_counter() - t0
     rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))  # noqa: E731
     return {
         "value": 1.0 / t32, "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
-        "sample": f"1 OS-EM slice-iteration ({ss} subsets) of one {n}^2 x {pa}-angle slice, FP32 compat "
+        "sample": f"1 OS-EM slice-iteration ({ss} subsets) of one {n_time}^2 x {pa}-angle slice, FP32 compat "
                   f"restatement of OSEM1/OSEM2 in C with OpenMP; {t32:.2f} s",
-        "fp64_oracle_seconds": t64,
     }, rel(gpu_slice0, c32), rel(gpu_slice0, c64)
 
 
@@ -88,11 +94,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=128)
-    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--workload", choices=("uniform", "mixed"), default="uniform")
+    ap.add_argument("--batch", type=int, default=128, help="uniform: slices in the batch")
+    ap.add_argument("--n", type=int, default=1024, help="uniform: slice side")
+    ap.add_argument("--energies", type=int, default=64, help="mixed: energies per slice")
+    ap.add_argument("--slices", type=int, default=128, help="mixed: slice positions")
     ap.add_argument("--pa", type=int, default=1200)
     ap.add_argument("--ss", type=int, default=10)
-    ap.add_argument("--per-slice", type=int, default=4, help="slices timed in the per-slice-launch baseline")
+    ap.add_argument("--per-slice", type=int, default=4, help="uniform: slices timed in the per-slice-launch baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -107,19 +116,31 @@ def main():
     import torch
 
     from ctrecon import Context
+    from ctrecon import phantom as ph
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    B, n, pa, ss = args.batch, args.n, args.pa, args.ss
+    pa, ss = args.pa, args.ss
+    na = pa // ss
     dev = torch.device("cuda:0")
-    log(f"generating {B} phantoms {n}^2 x {pa}")
-    ang, sinos = make_batch(B, n, pa, seed=2026, device=dev)
-    imgs = [torch.ones((n, n), dtype=torch.float32, device=dev) for _ in range(B)]
+    if args.workload == "uniform":
+        log(f"generating {args.batch} phantoms {args.n}^2 x {pa}")
+        ang, sinos = make_batch(args.batch, args.n, pa, seed=2026, device=dev)
+        sides = [args.n] * args.batch
+    else:
+        E, S = args.energies, args.slices
+        log(f"generating the mixed stack: {E} energies x {S} slices, sides {ph.C3_SIDES}, {pa} angles")
+        sides_s, ang, sinos = ph.mixed_energy_stack_torch(E, S, pa, seed=2026, device=dev)
+        sides = [sides_s[i // E] for i in range(E * S)]
+    B = len(sides)
+    imgs = [torch.ones((m, m), dtype=torch.float32, device=dev) for m in sides]
     torch.cuda.synchronize()
+    eq = float(sum(m * m for m in sides)) / 1024.0 ** 2        # 1024^2-equivalent slices
+    nmax = max(sides)
 
     ctx = Context(0)
-    ctx.set_geometry(n, n, pa, ss, angles_deg=ang)
+    ctx.set_geometry(sides[0], sides[0], pa, ss, angles_deg=ang)
     tune = None
     if not args.no_tune:
-        tune = ctx.autotune(n)
+        tune = ctx.autotune(nmax)     # a batched launch runs the schedule of its largest side
         log("autotune (fp rays/wave, fp waves/block, bp rows/thread):", tune)
 
     # first call: one iteration from the uniform start -> parity sample of slice 0
@@ -150,64 +171,89 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+        if args.workload == "mixed":
+            ctx.synchronize()
+            log(f"step {s + 1}/{args.steps} done")
     ctx.synchronize()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    value = B * args.steps / dt
-    log(f"batched: {value:.2f} slice-iters/s ({dt / args.steps * 1e3:.1f} ms per step)")
+    value = eq * args.steps / dt
+    log(f"batched: {value:.2f} (1024^2-equivalent) slice-iters/s ({dt / args.steps * 1e3:.1f} ms per step)")
 
     # per-slice launch baseline: own kernels, one slice per call, sync after every kernel
     pctx = Context(0)
-    pctx.set_geometry(n, n, pa, ss, angles_deg=ang)
-    if tune:
-        pctx.autotune(n)
     pctx.set_sync_each_kernel(True)
-    K = max(1, args.per_slice)
-    xs = [torch.ones((1, n, n), dtype=torch.float32, device=dev) for _ in range(K)]
-    ys = [sinos[i % B].view(1, pa, n) for i in range(K)]
-    pctx.osem_execution(xs[0], ys[0].clone(), 1)
-    pctx.synchronize()
-    t0 = time.perf_counter()
-    for i in range(K):
-        pctx.osem_execution(xs[i], ys[i], 1)
-    pctx.synchronize()
-    dtp = time.perf_counter() - t0
-    per_slice = K / dtp
-    log(f"per-slice launch: {per_slice:.2f} slice-iters/s")
+    if args.workload == "uniform":
+        sample = [i % B for i in range(max(1, args.per_slice))]
+    else:
+        sample = [sides.index(m) for m in sorted(set(sides))]     # one slice of each side
+    t_side = {}
+    for m in sorted(set(sides[i] for i in sample)):
+        pctx.set_geometry(m, m, pa, ss, angles_deg=ang)
+        if tune:
+            pctx.autotune(m)
+        for i in (i for i in sample if sides[i] == m):
+            x = torch.ones((1, m, m), dtype=torch.float32, device=dev)
+            y = sinos[i].view(1, pa, m)
+            pctx.osem_execution(x.clone(), y.clone(), 1)          # warm
+            pctx.synchronize()
+            t1 = time.perf_counter()
+            pctx.osem_execution(x, y.clone(), 1)
+            pctx.synchronize()
+            t_side.setdefault(m, []).append(time.perf_counter() - t1)
+    t_mean = {m: sum(v) / len(v) for m, v in t_side.items()}
+    per_slice = eq / sum(t_mean[m] for m in sides)
+    log(f"per-slice launch: {per_slice:.2f} (1024^2-equivalent) slice-iters/s")
 
-    na = pa // ss
     fp_kernel = "k_fp_lds" if (tune is None or tune[0] == 0) else "k_fp"
-    fp_flops = 14.0 * n * na * n * B            # 14 FLOP per bilinear sample, D = n samples per ray
-    fp_bytes = (4 * n * n + 8 * na * n) * B     # image once, measured rows in, ratio rows out
+    fp_flops = sum(14.0 * m * na * m for m in sides)          # 14 FLOP per bilinear sample, D = m samples per ray
+    fp_bytes = sum(4 * m * m + 8 * na * m for m in sides)    # image once, measured rows in, ratio rows out
     fp_tflops = fp_flops / (fp_ms * 1e-3) / 1e12
-    step_flops = (14.0 * n * pa * n + 2.0 * n * n * pa) * B
-    hbm_bytes = (12.0 * ss * n * n + 12.0 * n * pa) * B
+    step_flops = sum(14.0 * m * pa * m + 2.0 * m * m * pa for m in sides)
+    hbm_bytes = sum(12.0 * ss * m * m + 12.0 * m * pa for m in sides)
+    uniform_1024 = args.workload == "uniform" and (args.n, pa, ss) == FP_TRAFFIC_SHAPE
     roofline = {
         "bound": "valu", "achieved": round(fp_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": round(fp_tflops / PEAK_FP32_TFLOPS, 4),
-        "traffic": (round(FP_TRAFFIC_BYTES_PER_SLICE * B) if (n, pa, ss) == FP_TRAFFIC_SHAPE and fp_kernel == "k_fp_lds"
-                    else None),
+        "traffic": (round(FP_TRAFFIC_BYTES_PER_SLICE * B) if uniform_1024 and fp_kernel == "k_fp_lds" else None),
         "algorithmic_bytes": fp_bytes,
         "kernel": f"{fp_kernel}<FP_RATIO> (OSEM1: forward projection + ratio), one launch = one subset of every slice",
         "fp_ms_per_launch": round(fp_ms, 4), "bp_ms_per_launch": round(bp_ms, 4),
-        "whole_step_tflops": round(step_flops * value / B / 1e12, 3),
-        "whole_step_hbm_frac": round(hbm_bytes * value / B / (PEAK_HBM_GBS * 1e9), 5),
+        "whole_step_tflops": round(step_flops * args.steps / dt / 1e12, 3),
+        "whole_step_hbm_frac": round(hbm_bytes * args.steps / dt / (PEAK_HBM_GBS * 1e9), 5),
     }
+    if args.workload == "uniform":
+        workload = (f"batched OS-EM, {B} slices of {args.n}x{args.n}, {pa} projections over [0,180), {args.n} bins, "
+                    f"{ss} subsets; step = one OS-EM iteration over the batch")
+        cfg = {"workload": workload, "slices": B, "n": args.n, "pa": pa, "ss": ss, "autotune": tune}
+        unit = "slice-iters/s"
+        data = f"synthetic: {B} seeded 10-ellipse phantoms, analytic sinograms with Poisson noise at I0=1e4"
+    else:
+        counts = {m: sides.count(m) for m in sorted(set(sides))}
+        workload = (f"batched OS-EM over a mixed CT-XAFS stack: {args.energies} energies x {args.slices} slices "
+                    f"(slices per side {counts}), {pa} projections, {ss} subsets; step = one OS-EM iteration over "
+                    f"the stack; value in 1024^2-equivalent slice-iterations/s")
+        cfg = {"workload": workload, "energies": args.energies, "slices": args.slices, "side_counts": counts,
+               "pa": pa, "ss": ss, "autotune": tune}
+        unit = "1024^2-equivalent slice-iters/s"
+        data = ("synthetic: seeded 10-ellipse phantoms per slice, sigmoid absorption edge over energies, analytic "
+                "sinograms with Poisson noise at I0=1e4")
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "slice-iters/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": f"synthetic: {B} seeded 10-ellipse phantoms, analytic sinograms with Poisson noise at I0=1e4",
-        "config": {"workload": f"batched OS-EM, {B} slices of {n}x{n}, {pa} projections over [0,180), {n} bins, "
-                               f"{ss} subsets; step = one OS-EM iteration over the batch",
-                   "slices": B, "n": n, "pa": pa, "ss": ss, "autotune": tune},
-        "per_slice_launch": {"value": round(per_slice, 3), "unit": "slice-iters/s", "slices_timed": K},
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": data, "config": cfg,
+        "per_slice_launch": {"value": round(per_slice, 3), "unit": unit, "slices_timed": len(sample)},
         "batched_over_per_slice": round(value / per_slice, 3),
         "roofline": roofline,
     }
     if not args.no_cpu:
         log("CPU baseline / oracle check on slice 0 ...")
-        cb, e32, e64 = cpu_baseline(n, pa, ss, ang, sinos[0].cpu().numpy(), gpu_slice0)
+        i_time = sides.index(1024) if 1024 in sides else 0
+        cb, e32, e64 = cpu_baseline(sides[0], pa, ss, ang, sinos[0].cpu().numpy(), gpu_slice0,
+                                    sides[i_time], sinos[i_time].cpu().numpy())
+        if args.workload == "mixed":
+            cb["value"] *= sides[i_time] ** 2 / 1024.0 ** 2
+            cb["unit"] = unit
         out["cpu_baseline"] = cb
         out["relerr_vs_fp32_compat"] = e32
         out["relerr_vs_fp64"] = e64

@@ -195,20 +195,57 @@ ctr_status upload_set(ctr_ctx *c, SliceSet &S)
     for (int k = 0; k < 6; ++k) S.hmap[k].clear();
     for (int s = 0; s < n; ++s) {
         SliceDev &d = S.h[s];
-        const int cnt[6] = {fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays), bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny),
-                            pad_blocks(d.ny, d.ipitch), ew_blocks(d.nx, d.ny), 0};
-        for (int k = 0; k < 6; ++k) {
+        const int cnt[6] = {0, bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny), pad_blocks(d.ny, d.ipitch),
+                            ew_blocks(d.nx, d.ny), 0};
+        for (int k = 1; k < 6; ++k) {
             d.blk[k] = (int)S.hmap[k].size();
             S.hmap[k].insert(S.hmap[k].end(), (size_t)cnt[k], s);
         }
+        d.blk[G_FP] = 0;
+    }
+    // Forward projection: a block of a slice with D samples per ray does work
+    // proportional to D, so equal block counts are not equal work.  The block
+    // sequence is cut into 8 segments of equal work (a slice may straddle two),
+    // logical segment x runs on XCD x (xcd_remap), and the segments are padded
+    // to one length with empty entries.  Entries are (slice, block within the
+    // slice) pairs; slice -1 is empty.
+    {
+        std::vector<int> cnt(n);
+        double total = 0.0;
+        for (int s = 0; s < n; ++s) {
+            const SliceDev &d = S.h[s];
+            cnt[s] = fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays);
+            total += (double)cnt[s] * std::max(d.nx, d.ny);
+        }
+        std::vector<std::vector<std::pair<int, int>>> seg(8);
+        double acc = 0.0;
+        int x = 0;
+        for (int s = 0; s < n; ++s) {
+            const double w = std::max(S.h[s].nx, S.h[s].ny);
+            for (int b = 0; b < cnt[s]; ++b) {
+                while (x < 7 && acc + 0.5 * w > total * (x + 1) / 8.0) ++x;
+                seg[x].push_back({s, b});
+                acc += w;
+            }
+        }
+        size_t per = 0;
+        for (const auto &v : seg) per = std::max(per, v.size());
+        std::vector<int> &fm = S.hmap[G_FP];
+        fm.assign(2 * 8 * per, -1);
+        for (int k = 0; k < 8; ++k)
+            for (size_t i = 0; i < seg[k].size(); ++i) {
+                fm[2 * (k * per + i)] = seg[k][i].first;
+                fm[2 * (k * per + i) + 1] = seg[k][i].second;
+            }
     }
     HIPTRY(c, S.dsl.ensure(sizeof(SliceDev) * std::max(n, 1)));
     HIPTRY(c, hipMemcpyAsync(S.dsl.p, S.h.data(), sizeof(SliceDev) * n, hipMemcpyHostToDevice, c->stream));
     for (int k = 0; k < 6; ++k) {
-        S.nb[k] = (int)S.hmap[k].size();
-        HIPTRY(c, S.dmap[k].ensure(sizeof(int) * std::max(S.nb[k], 1)));
-        if (S.nb[k])
-            HIPTRY(c, hipMemcpyAsync(S.dmap[k].p, S.hmap[k].data(), sizeof(int) * S.nb[k], hipMemcpyHostToDevice,
+        const size_t entries = S.hmap[k].size();
+        S.nb[k] = (int)(k == G_FP ? entries / 2 : entries);
+        HIPTRY(c, S.dmap[k].ensure(sizeof(int) * std::max(entries, (size_t)2)));
+        if (entries)
+            HIPTRY(c, hipMemcpyAsync(S.dmap[k].p, S.hmap[k].data(), sizeof(int) * entries, hipMemcpyHostToDevice,
                                      c->stream));
     }
     return CTR_OK;
@@ -329,7 +366,7 @@ void time_end(ctr_ctx *c, int which)
 
 ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer, int geo_power)
 {
-    FpParams p{sub, c->g.ss, c->g.pa / c->g.ss, sub_from_layer, geo_power};
+    FpParams p{sub, c->g.ss, c->g.pa / c->g.ss, sub_from_layer, geo_power, 0};
     time_begin(c, 0);
     launch_fp(var, S.fp_wpb, S.fp_rays, launch_of(c, S, G_FP), p, c->stream);
     POST(c, "forward projection");
@@ -1156,7 +1193,7 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
             S.fp_rays = rays;
             S.bp_rows = 4;
             TRY(upload_set(c, S));
-            FpParams p{0, c->g.ss, na, 0, 0};
+            FpParams p{0, c->g.ss, na, 0, 0, 0};
             const double ms = timeit([&] { launch_fp(FP_RATIO, w, rays, launch_of(c, S, G_FP), p, c->stream); });
             if (ms < best) { best = ms; best_t.fp_wpb = w; best_t.fp_rays = rays; }
         }

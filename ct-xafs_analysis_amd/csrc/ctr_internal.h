@@ -38,7 +38,8 @@ struct SliceDev {
     int nx, ny, px;
     int ipitch, tpitch, rpitch;
     int layer;           // index in its stack (subset index for the power iteration)
-    int blk[6];          // first logical block of this slice for each grid kind
+    int blk[6];          // first logical block of this slice for each grid kind (the
+                         // G_FP map holds (slice, block) pairs instead)
 };
 
 enum GridKind { G_FP = 0, G_BP = 1, G_TR = 2, G_PAD = 3, G_EW = 4, G_SPARE = 5 };
@@ -51,12 +52,14 @@ enum BpVar { BP_OSEM2 = 0, BP_AART2 = 1, BP_INIT = 2, BP_FISTA = 3, BP_POWER = 4
 struct Launch {
     const SliceDev *sl;   // device descriptor array
     const int *bmap;      // device logical-block -> slice map for this grid kind
+                          // (G_FP: (slice, block within slice) pairs, slice -1 = empty)
     const float2 *cs;     // device [pa] (cos, sin) of the float angle
     int nblocks;
 };
 
 struct FpParams {
     int sub, ss, na, sub_from_layer, geo_power;
+    int nblk;   // logical blocks of the launch (set by launch_fp; the grid may be 2-D)
 };
 
 struct BpParams {

@@ -195,3 +195,35 @@ def transmission_noise_torch(p, i0: float, seed: int):
     counts = torch.poisson(i0 * torch.exp(-p), generator=g)
     trans = torch.where(counts > 0, i0 / counts, torch.full_like(counts, 1.0e7))
     return torch.log(torch.clamp(trans, 1.0e-5, 1.0e7))
+
+
+C3_SIDES = (256, 384, 512, 768, 1024)
+
+
+def mixed_stack_sides(n_slices: int, seed: int, sides=C3_SIDES) -> list:
+    """Seeded side of each slice of an uneven stack (shared by all energies)."""
+    return [int(sides[np.random.default_rng(stream_seed(seed, s, 0)).integers(len(sides))]) for s in range(n_slices)]
+
+
+def mixed_energy_stack_torch(n_energy: int, n_slices: int, pa: int, seed: int, *, n_ellipses: int = 10,
+                             i0: float | None = 1.0e4, sides=C3_SIDES, slice_sides=None, device="cuda"):
+    """The batched CT-XAFS stack: slice s has a seeded side from `sides` and a
+    seeded ellipse set shared by all energies; each ellipse's attenuation
+    follows sigmoid_edge_mu over the energies; Poisson noise per (slice,
+    energy).  slice_sides, when given, fixes the side of every slice instead.
+    Returns (sides, angles, sinograms) with sinograms a list in slice-major
+    order (index s * n_energy + e) of float32 tensors [pa][n]."""
+    ang = uniform_angles(pa)
+    n_of = list(slice_sides) if slice_sides is not None else mixed_stack_sides(n_slices, seed, sides)
+    sinos = [None] * (n_slices * n_energy)
+    for s in range(n_slices):
+        rng = np.random.default_rng(stream_seed(seed, s, 1))
+        ell = random_ellipses(n_ellipses, rng)
+        mus = sigmoid_edge_mu(ell.mu, n_energy, rng)
+        sets = [ell.with_mu(mus[e]) for e in range(n_energy)]
+        p = line_integrals_torch(sets, n_of[s], ang, n_of[s], device=device)
+        for e in range(n_energy):
+            y = p[e] if i0 is None else transmission_noise_torch(p[e], i0, stream_seed(seed, s, e + 2))
+            sinos[s * n_energy + e] = y.float().contiguous()
+        del p
+    return n_of, ang, sinos
