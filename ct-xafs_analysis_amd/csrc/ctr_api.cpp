@@ -377,7 +377,7 @@ ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer,
 ctr_status run_bp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer, float alpha, const float *L,
                   int to_aux)
 {
-    BpParams p{sub, c->g.ss, c->g.pa / c->g.ss, c->g.pa, sub_from_layer, alpha, L, to_aux};
+    BpParams p{sub, c->g.ss, c->g.pa / c->g.ss, c->g.pa, sub_from_layer, alpha, L, to_aux, 0};
     time_begin(c, 1);
     launch_bp(var, S.bp_rows, launch_of(c, S, G_BP), p, c->stream);
     POST(c, "backprojection");
@@ -490,6 +490,13 @@ extern "C" ctr_status ctr_set_stream(ctr_ctx *c, void *s)
 {
     if (!c) return CTR_INVALID_ARGUMENT;
     c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_get_stream(const ctr_ctx *c, void **s)
+{
+    if (!c || !s) return CTR_INVALID_ARGUMENT;
+    *s = (void *)c->stream;
     return CTR_OK;
 }
 
@@ -1204,7 +1211,7 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
         S.fp_rays = best_t.fp_rays;
         S.bp_rows = r;
         TRY(upload_set(c, S));
-        BpParams p{0, c->g.ss, na, c->g.pa, 0, 0.0f, nullptr, 0};
+        BpParams p{0, c->g.ss, na, c->g.pa, 0, 0.0f, nullptr, 0, 0};
         const double ms = timeit([&] { launch_bp(BP_OSEM2, r, launch_of(c, S, G_BP), p, c->stream); });
         if (ms < best) { best = ms; best_t.bp_rows = r; }
     }

@@ -67,6 +67,7 @@ struct BpParams {
     float alpha;
     const float *L;       // L2norm (FISTA backprojection) or null
     int to_aux;           // write the result to aux instead of the padded image
+    int nblk;             // logical blocks of the launch (set by launch_bp)
 };
 
 // ---- launchers (ctr_kernels.hip) ----

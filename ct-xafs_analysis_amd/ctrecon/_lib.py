@@ -47,6 +47,7 @@ _SIGS = {
     "ctr_create": [_I, C.POINTER(_P)],
     "ctr_destroy": [_P],
     "ctr_set_stream": [_P, _P],
+    "ctr_get_stream": [_P, _P],
     "ctr_synchronize": [_P],
     "ctr_set_sync_each_kernel": [_P, _I],
     "ctr_set_geometry": [_P, C.POINTER(Geometry), _P, _P],
@@ -125,8 +126,9 @@ def _ptr(t):
 
 class Context:
     """One library context (device + stream).  Methods take float32 GPU tensors
-    and run on the context stream; call synchronize() before reading results
-    from other streams."""
+    and run on the context stream, ordered after the work queued on torch's
+    current stream and before the work queued there afterwards (as a torch
+    operation would be).  synchronize() waits for the context stream."""
 
     def __init__(self, device: int = 0, use_torch_stream: bool = False):
         import torch
@@ -137,9 +139,34 @@ class Context:
         if st != CTR_OK:
             raise CtrError(st, f"ctr_create(device={device})")
         self.h = h
+        self.device = device
         self.geom = None
+        self._ext = None
         if use_torch_stream:
             self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+        else:
+            self._bind_stream()
+
+    def _bind_stream(self):
+        """Wraps the library stream so calls can be ordered with torch's."""
+        s = C.c_void_p()
+        self._chk(self.lib.ctr_get_stream(self.h, C.byref(s)), "ctr_get_stream")
+        cur = self._torch.cuda.current_stream(self.device)
+        self._ext = None if s.value == cur.cuda_stream else \
+            self._torch.cuda.ExternalStream(s.value, device=self.device)
+
+    def _run(self, fn, args, what):
+        """Calls a library entry point in stream order with torch: the library
+        stream first waits for the work already queued on torch's current
+        stream (inputs written there), and torch's current stream then waits
+        for the library stream (outputs read there)."""
+        cur = self._torch.cuda.current_stream(self.device)
+        if self._ext is not None:
+            self._ext.wait_stream(cur)
+        st = fn(*args)
+        if self._ext is not None:
+            cur.wait_stream(self._ext)
+        self._chk(st, what)
 
     def _chk(self, st, what):
         if st != CTR_OK:
@@ -159,12 +186,13 @@ class Context:
 
     def set_stream(self, stream_handle):
         self._chk(self.lib.ctr_set_stream(self.h, C.c_void_p(stream_handle)), "ctr_set_stream")
+        self._bind_stream()
 
     def synchronize(self):
-        self._chk(self.lib.ctr_synchronize(self.h), "ctr_synchronize")
+        self._run(self.lib.ctr_synchronize, (self.h,), "ctr_synchronize")
 
     def set_sync_each_kernel(self, on: bool):
-        self._chk(self.lib.ctr_set_sync_each_kernel(self.h, int(bool(on))), "ctr_set_sync_each_kernel")
+        self._run(self.lib.ctr_set_sync_each_kernel, (self.h, int(bool(on)),), "ctr_set_sync_each_kernel")
 
     def set_geometry(self, nx, px, pa, ss=1, *, ny=None, zp=None, mode=6, correction_mode=0, wt1=1.0, wt2=0.01,
                      cs_lambda=1e-4, angles_deg=None, sub=None):
@@ -181,38 +209,38 @@ class Context:
         if sub is not None:
             sub = np.ascontiguousarray(sub, dtype=np.int32)
             subp = sub.ctypes.data_as(_P)
-        self._chk(self.lib.ctr_set_geometry(self.h, C.byref(g), ang.ctypes.data_as(_P), subp), "ctr_set_geometry")
+        self._run(self.lib.ctr_set_geometry, (self.h, C.byref(g), ang.ctypes.data_as(_P), subp,), "ctr_set_geometry")
         self.geom = g
         return g
 
     # ---- drivers ----
     def osem_execution(self, img, prj, it, prj_correction=True):
-        self._chk(self.lib.ctr_osem_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction)),
+        self._run(self.lib.ctr_osem_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction),),
                   "ctr_osem_execution")
 
     def fista_osem_execution(self, img, prj, it, prj_correction=True):
-        self._chk(self.lib.ctr_fista_osem_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it,
-                                                    int(prj_correction)), "ctr_fista_osem_execution")
+        self._run(self.lib.ctr_fista_osem_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it,
+                                                    int(prj_correction),), "ctr_fista_osem_execution")
 
     def aart_execution(self, img, prj, it):
-        self._chk(self.lib.ctr_aart_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], it), "ctr_aart_execution")
+        self._run(self.lib.ctr_aart_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it,), "ctr_aart_execution")
 
     def fbp_execution(self, img, prj, prj_correction=True):
-        self._chk(self.lib.ctr_fbp_execution(self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction)),
+        self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction),),
                   "ctr_fbp_execution")
 
     def power_iteration(self, L, trials=30):
-        self._chk(self.lib.ctr_power_iteration(self.h, _ptr(L), trials), "ctr_power_iteration")
+        self._run(self.lib.ctr_power_iteration, (self.h, _ptr(L), trials,), "ctr_power_iteration")
 
     def fista_execution(self, L, img, prj, it, prj_correction=True):
-        self._chk(self.lib.ctr_fista_execution(self.h, _ptr(L), _ptr(img), _ptr(prj), img.shape[0], it,
-                                               int(prj_correction)), "ctr_fista_execution")
+        self._run(self.lib.ctr_fista_execution, (self.h, _ptr(L), _ptr(img), _ptr(prj), img.shape[0], it,
+                                               int(prj_correction),), "ctr_fista_execution")
 
     # ---- single operators ----
     def op(self, name, *args):
         f = getattr(self.lib, "ctr_op_" + name)
         conv = [(_ptr(a) if (a is None or isinstance(a, self._torch.Tensor)) else a) for a in args]
-        self._chk(f(self.h, *conv), "ctr_op_" + name)
+        self._run(f, (self.h, *conv), "ctr_op_" + name)
 
     # ---- batched ----
     def osem_batched(self, imgs, prjs, it, use_graph=True):
@@ -221,35 +249,35 @@ class Context:
         for i, (im, pr) in enumerate(zip(imgs, prjs)):
             arr[i] = Slice(int(im.shape[-1]), _ptr(im).value, _ptr(pr).value)
         self._keep = (imgs, prjs, arr)
-        self._chk(self.lib.ctr_osem_batched(self.h, arr, n, it, int(use_graph)), "ctr_osem_batched")
+        self._run(self.lib.ctr_osem_batched, (self.h, arr, n, it, int(use_graph),), "ctr_osem_batched")
 
     def autotune(self, n):
         """Returns (rays per projection wave, waves per projection block, rows per backprojection thread)."""
         rays, w, r = C.c_int(), C.c_int(), C.c_int()
-        self._chk(self.lib.ctr_autotune(self.h, n, C.byref(rays), C.byref(w), C.byref(r)), "ctr_autotune")
+        self._run(self.lib.ctr_autotune, (self.h, n, C.byref(rays), C.byref(w), C.byref(r),), "ctr_autotune")
         return rays.value, w.value, r.value
 
     def set_schedule(self, n, fp_rays, fp_waves, bp_rows):
-        self._chk(self.lib.ctr_set_schedule(self.h, n, fp_rays, fp_waves, bp_rows), "ctr_set_schedule")
+        self._run(self.lib.ctr_set_schedule, (self.h, n, fp_rays, fp_waves, bp_rows,), "ctr_set_schedule")
 
     def set_kernel_timing(self, on):
-        self._chk(self.lib.ctr_set_kernel_timing(self.h, int(bool(on))), "ctr_set_kernel_timing")
+        self._run(self.lib.ctr_set_kernel_timing, (self.h, int(bool(on)),), "ctr_set_kernel_timing")
 
     def kernel_timing(self):
         a, b, na, nb = C.c_double(), C.c_double(), C.c_int(), C.c_int()
-        self._chk(self.lib.ctr_kernel_timing(self.h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb)),
+        self._run(self.lib.ctr_kernel_timing, (self.h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb),),
                   "ctr_kernel_timing")
         return a.value, b.value, na.value, nb.value
 
     # ---- differentiable model ----
     def align_forward(self, img, shifts, center, prj):
-        self._chk(self.lib.ctr_align_forward(self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj),
-                                             img.shape[0]), "ctr_align_forward")
+        self._run(self.lib.ctr_align_forward, (self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj),
+                                             img.shape[0],), "ctr_align_forward")
 
     def align_adjoint(self, prj, shifts, center, img):
-        self._chk(self.lib.ctr_align_adjoint(self.h, _ptr(prj), _ptr(shifts), float(center), _ptr(img),
-                                             img.shape[0]), "ctr_align_adjoint")
+        self._run(self.lib.ctr_align_adjoint, (self.h, _ptr(prj), _ptr(shifts), float(center), _ptr(img),
+                                             img.shape[0],), "ctr_align_adjoint")
 
     def align_grad_shift(self, img, gprj, shifts, center, gshift):
-        self._chk(self.lib.ctr_align_grad_shift(self.h, _ptr(img), _ptr(gprj), _ptr(shifts), float(center),
-                                                _ptr(gshift), img.shape[0]), "ctr_align_grad_shift")
+        self._run(self.lib.ctr_align_grad_shift, (self.h, _ptr(img), _ptr(gprj), _ptr(shifts), float(center),
+                                                _ptr(gshift), img.shape[0],), "ctr_align_grad_shift")

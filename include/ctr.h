@@ -76,6 +76,10 @@ const char *ctr_status_string(ctr_status status);
 /* Uses `hip_stream` (a hipStream_t) instead of the context's own stream;
  * NULL restores the context stream. */
 ctr_status ctr_set_stream(ctr_ctx *ctx, void *hip_stream);
+/* Writes the hipStream_t the context's work runs on (its own stream, or the
+ * one given to ctr_set_stream), so a caller can order its own streams with it.
+ * No reference counterpart (an OpenCL queue is the caller's object there). */
+ctr_status ctr_get_stream(const ctr_ctx *ctx, void **hip_stream);
 ctr_status ctr_synchronize(ctr_ctx *ctx);
 
 /* 1: synchronise after every kernel launch, as GPU.cpp does with finish()

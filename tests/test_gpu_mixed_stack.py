@@ -39,3 +39,24 @@ def test_mixed_stack_reduced(ctx):
             e = relerr(imgs[i].cpu().numpy()[None], want)
             print(f"[parity] mixed stack side {m}: relerr vs FP32 compat {e:.3e}", flush=True)
             assert e < 1e-5
+
+
+def test_multi_row_grids(ctx):
+    """40 slices of 1024^2: the forward projection (76800 blocks), padding and
+    elementwise grids pass 2^16 blocks and launch as 2-D grids; every slice
+    still equals its own OSEM_execution."""
+    import torch
+    from ctrecon import phantom as ph
+    n, pa, ss, it, B = 1024, 1200, 10, 1, 40
+    _, ang, sinos = ph.mixed_energy_stack_torch(1, 1, pa, seed=5, slice_sides=[n])
+    ctx.set_geometry(n, n, pa, ss, angles_deg=ang)
+    ctx.set_schedule(n, 0, 4, 8)
+    x = torch.ones((1, n, n), device="cuda")
+    say("per-slice 1024")
+    ctx.osem_execution(x, sinos[0].view(1, pa, n).clone(), it)
+    imgs = [torch.ones((n, n), device="cuda") for _ in range(B)]
+    say("batched", B, "x 1024")
+    ctx.osem_batched(imgs, [sinos[0]] * B, it, use_graph=False)
+    ctx.synchronize()
+    for i in range(B):
+        assert torch.equal(imgs[i], x[0]), i
