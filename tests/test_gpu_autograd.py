@@ -1,0 +1,130 @@
+"""torch.autograd front end of the alignment model (ctrecon.autograd) and the
+joint image + shift + centre FISTA-TV (ctrecon.joint).
+
+Gradient checks on a 64^2 x 90-projection subproblem: image, shift and centre
+gradients of a weighted sum of the projection against the FP64 oracle's
+adjoint and shift gradient, the centre gradient against FP64 central finite
+differences, the dot-product test through torch.autograd.grad, and chunked
+stacks against unchunked ones.  Joint recovery on a 256^2 x 360-projection
+case with shifts N(0, 1.5) px and a centre offset U(-3, 3) px."""
+import numpy as np
+import pytest
+
+import oracle as O
+from _util import relerr, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ctx, n=64, pa=90, seed=11):
+    rng = np.random.default_rng(seed)
+    ang = (np.arange(pa) * (180.0 / pa) + 0.37).astype(np.float32)
+    ctx.set_geometry(n, n, pa, 1, angles_deg=ang)
+    return rng, ang, O.geom(n, n, pa, ss=1)
+
+
+def test_gradients_vs_fp64_oracle_and_finite_differences(ctx):
+    import torch
+    from ctrecon import phantom as ph
+    from ctrecon.autograd import project
+    rng, ang, g = _setup(ctx)
+    img = np.stack([ph.rasterize(ph.random_ellipses(8, rng), 64, supersample=2) for _ in range(2)])
+    img = img.astype(np.float32)
+    sh = rng.normal(0, 1.5, 90).astype(np.float32)
+    w = rng.normal(0, 1, (2, 90, 64)).astype(np.float32)
+    center = 0.7
+    x = torch.from_numpy(img).cuda().requires_grad_(True)
+    s = torch.from_numpy(sh).cuda().requires_grad_(True)
+    c = torch.tensor(center, device="cuda", requires_grad=True)
+    wt = torch.from_numpy(w).cuda()
+    say("autograd forward/backward")
+    loss = (wt * project(ctx, x, s, c)).sum()
+    loss.backward()
+    gx = O.align_adjoint(g, ang, w, sh, center, dtype=np.float64)
+    gs = O.align_grad(g, ang, img, w, sh, center, dtype=np.float64)
+    ex, es = relerr(x.grad.cpu().numpy(), gx), relerr(s.grad.cpu().numpy(), gs)
+    print(f"[grad] image {ex:.2e} shifts {es:.2e}", flush=True)
+    assert ex < 1e-5 and es < 1e-4
+    # centre: FP64 central differences of the oracle forward
+    h = 1e-6
+    f = lambda cc: float((w.astype(np.float64) * O.align_forward(g, ang, img, sh, cc, dtype=np.float64)).sum())  # noqa: E731
+    fd = (f(center + h) - f(center - h)) / (2 * h)
+    ec = abs(float(c.grad) - fd) / abs(fd)
+    print(f"[grad] centre {float(c.grad):.6e} fd {fd:.6e} rel {ec:.2e}", flush=True)
+    assert ec < 1e-3
+    assert abs(float(c.grad) - float(s.grad.sum())) <= 1e-6 * abs(float(c.grad)) + 1e-6
+
+
+def test_dot_product_through_autograd(ctx):
+    import torch
+    from ctrecon.autograd import project
+    rng, ang, g = _setup(ctx)
+    sh = torch.from_numpy(rng.normal(0, 1.5, 90).astype(np.float32)).cuda()
+    c = torch.tensor(-0.4, device="cuda")
+    for trial in range(3):
+        v = torch.from_numpy(rng.uniform(0, 1, (1, 64, 64)).astype(np.float32)).cuda().requires_grad_(True)
+        u = torch.from_numpy(rng.uniform(0, 1, (1, 90, 64)).astype(np.float32)).cuda()
+        say("autograd dot test", trial)
+        av = project(ctx, v, sh, c)
+        (atu,) = torch.autograd.grad(av, v, grad_outputs=u)
+        lhs = float((av.double() * u.double()).sum())
+        rhs = float((v.detach().double() * atu.double()).sum())
+        rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+        print(f"[dot] autograd <Av,u>={lhs:.9e} <v,A^T u>={rhs:.9e} rel={rel:.2e}", flush=True)
+        assert rel < 1e-6
+
+
+def test_chunked_stack_matches(ctx):
+    import torch
+    from ctrecon.autograd import project
+    rng, ang, g = _setup(ctx)
+    img = torch.from_numpy(rng.uniform(0, 1, (5, 64, 64)).astype(np.float32)).cuda()
+    sh0 = torch.from_numpy(rng.normal(0, 1, 90).astype(np.float32)).cuda()
+    w = torch.from_numpy(rng.normal(0, 1, (5, 90, 64)).astype(np.float32)).cuda()
+    res = []
+    for chunk in (0, 2):
+        x = img.clone().requires_grad_(True)
+        s = sh0.clone().requires_grad_(True)
+        c = torch.tensor(0.3, device="cuda", requires_grad=True)
+        say("chunk", chunk)
+        p = project(ctx, x, s, c, chunk=chunk)
+        (w * p).sum().backward()
+        res.append((p.detach(), x.grad, s.grad, c.grad))
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+    assert torch.allclose(res[0][2], res[1][2], rtol=1e-5, atol=1e-5 * float(res[0][2].abs().max()))
+    assert abs(float(res[0][3] - res[1][3])) <= 1e-5 * abs(float(res[0][3]))
+
+
+def test_joint_alignment_recovery(ctx):
+    """Shifts + centre are identifiable only modulo the translation components
+    {cos theta, sin theta} of the offsets (moving the image moves every
+    projection by x cos theta - y sin theta), so the offset error is measured
+    after removing its least-squares fit on them."""
+    import torch
+    from ctrecon import phantom as ph
+    from ctrecon.joint import joint_fista_tv
+    n, pa = 256, 360
+    rng = np.random.default_rng(4)
+    shifts = rng.normal(0, 1.5, pa)
+    center = rng.uniform(-3, 3)
+    _, sino, ang = ph.make_slice(n, pa, 10, 321, shifts=shifts, center=center)
+    ctx.set_geometry(n, n, pa, 1, angles_deg=ang)
+    y = torch.from_numpy(sino[None].copy()).cuda()
+    th = np.deg2rad(ang.astype(np.float64))
+    B = np.stack([np.cos(th), np.sin(th)], 1)
+
+    def off_err(off):
+        e = off - (shifts + center)
+        coef, *_ = np.linalg.lstsq(B, e, rcond=None)
+        return float(np.sqrt(np.mean((e - B @ coef) ** 2)))
+
+    say("joint FISTA-TV 256^2 x 360")
+    x, s, c, hist = joint_fista_tv(ctx, y, iters=100, tv=1e-3)
+    _, _, _, h0 = joint_fista_tv(ctx, y, iters=100, tv=1e-3, align=False)
+    e0 = off_err(np.zeros(pa))
+    e1 = off_err((s + c).cpu().numpy().astype(np.float64))
+    print(f"[joint] offset error mod translation {e0:.3f} -> {e1:.3f} px; data term {h0[-1]:.3e} (no alignment)"
+          f" -> {hist[-1]:.3e}", flush=True)
+    assert e1 < 0.3 and e1 < 0.2 * e0
+    assert hist[-1] < 0.2 * h0[-1]
+    assert float(x.min()) >= 1e-6 - 1e-12
