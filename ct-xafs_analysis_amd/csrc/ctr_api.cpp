@@ -111,7 +111,7 @@ struct ctr_ctx {
     DevBuf cs, cs_corr, tw;
     int tw_zp = 0;
     // stack workspace
-    DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull;
+    DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull, w_min;
     SliceSet set_a, set_b;
     // autotune table: n -> schedule
     std::map<int, Tune> tune;
@@ -699,6 +699,30 @@ extern "C" ctr_status ctr_fbp_execution(ctr_ctx *c, float *img, float *prj, int 
                   c->stream);
     POST(c, "backProjectionFBP");
     return CTR_OK;
+}
+
+// hybrid_thread, GPU.cpp:1459-1580 (mode 8): FBP_execution with the
+// projection correction, the base-up of the FBP image (FBPcorrection = 1:
+// findMinimumX/Y + baseUp, GPU.cpp:1538-1566), then OSEM_execution or
+// FISTA_OSEM_execution from that image without a second correction.  The
+// reference's minimum covers floor(n / work-group size) * work-group size
+// columns and rows, the whole layer whenever the sizes are multiples of its
+// work-group size; here it is always the whole layer.
+extern "C" ctr_status ctr_hybrid_execution(ctr_ctx *c, float *img, float *prj, int dN, int it, int prj_correction,
+                                           int baseup_order, int cs)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    if (baseup_order < 0 || baseup_order > 64) return fail(c, CTR_INVALID_ARGUMENT, "base-up order %d", baseup_order);
+    TRY(ctr_fbp_execution(c, img, prj, dN, prj_correction));
+    const ctr_geometry &g = c->g;
+    const size_t plane = (size_t)g.nx * g.ny;
+    HIPTRY(c, c->w_partial.ensure(sizeof(float) * l2_partial_count(dN)));
+    HIPTRY(c, c->w_min.ensure(sizeof(float) * (size_t)dN));
+    launch_layer_min(img, dN, plane, c->w_partial.f(), c->w_min.f(), c->stream);
+    POST(c, "findMinimumX/Y");
+    launch_baseup(img, c->w_min.f(), dN, plane, baseup_order, c->stream);
+    POST(c, "baseUp");
+    return cs ? ctr_fista_osem_execution(c, img, prj, dN, it, 0) : ctr_osem_execution(c, img, prj, dN, it, 0);
 }
 
 // powerIteration, GPU.cpp:2079-2188: ss layers, layer s = subset s, filled

@@ -84,6 +84,8 @@ void launch_tv(int fista, const float *x, const float *v, const float *b, float 
                float *b_new, int out_padded, int ipitch, int nz, int nx, int ny, const float *L,
                int sub, double lambda_fista, hipStream_t stream);
 void launch_l2(const float *src, int nz, size_t n, float *partial, float *l2, hipStream_t stream);
+void launch_layer_min(const float *src, int nz, size_t n, float *partial, float *mins, hipStream_t stream);
+void launch_baseup(float *img, const float *mins, int nz, size_t plane, int order, hipStream_t stream);
 void launch_power3(const float *cnd, const float *l2, float *pimg, int nz, int nx, int ny, int ipitch,
                    hipStream_t stream);
 void launch_fbp_filter(const float *prj, float *frows, int nz, int pa, int px, int rpitch, int zp,

@@ -47,6 +47,7 @@ _SIGS = {
     "ctr_create": [_I, C.POINTER(_P)],
     "ctr_destroy": [_P],
     "ctr_set_stream": [_P, _P],
+    "ctr_hybrid_execution": [_P, _P, _P, _I, _I, _I, _I, _I],
     "ctr_get_stream": [_P, _P],
     "ctr_synchronize": [_P],
     "ctr_set_sync_each_kernel": [_P, _I],
@@ -224,6 +225,10 @@ class Context:
 
     def aart_execution(self, img, prj, it):
         self._run(self.lib.ctr_aart_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it,), "ctr_aart_execution")
+
+    def hybrid_execution(self, img, prj, it, prj_correction=True, baseup_order=4, cs=False):
+        self._run(self.lib.ctr_hybrid_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction),
+                                                  baseup_order, int(bool(cs))), "ctr_hybrid_execution")
 
     def fbp_execution(self, img, prj, prj_correction=True):
         self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction),),

@@ -268,10 +268,6 @@ bool validate(const Params &p, std::string *err)
         set_err(err, "reconstruction method " + std::to_string(p.mode) + " is not 1-8");
         return false;
     }
-    if (p.mode == 8) {
-        set_err(err, "method 8 (FBP + OS-EM hybrid) is not provided by this library");
-        return false;
-    }
     if (p.ox != p.nx || p.oy != p.ny) {
         set_err(err, "output size must equal the reconstruction size (ROI extension is not provided)");
         return false;
@@ -281,13 +277,17 @@ bool validate(const Params &p, std::string *err)
         return false;
     }
     const int ss = method_subsets(p);
-    if (p.mode != 7 && (ss < 1 || p.pa % ss != 0)) {
+    if (p.mode != 7 && (ss < 1 || p.pa % ss != 0)) {   // FBP alone has no subsets
         set_err(err, "the projection count " + std::to_string(p.pa) + " is not a multiple of the subset count " +
                          std::to_string(ss));
         return false;
     }
-    if (p.mode == 7 && (p.zp < p.px || (p.zp & (p.zp - 1)) != 0)) {
+    if ((p.mode == 7 || p.mode == 8) && (p.zp < p.px || (p.zp & (p.zp - 1)) != 0)) {
         set_err(err, "the zero-padding size must be a power of two >= the detector width");
+        return false;
+    }
+    if (p.mode == 8 && (p.baseupOrder < 0 || p.baseupOrder > 64)) {
+        set_err(err, "base-up order must be 0-64");
         return false;
     }
     if (p.correctionMode < 0 || p.correctionMode > 4) {

@@ -22,7 +22,7 @@ namespace ctrhost {
 
 namespace {
 
-enum class Method { OSEM, FISTA_OSEM, AART, FISTA, FBP };
+enum class Method { OSEM, FISTA_OSEM, AART, FISTA, FBP, HYBRID };
 
 // GPU.cpp:36-93: the method a mode runs.
 Method method_of(const Params &p)
@@ -31,6 +31,7 @@ Method method_of(const Params &p)
     case 1:
     case 3: return p.CSitBool ? Method::FISTA : Method::AART;
     case 7: return Method::FBP;
+    case 8: return Method::HYBRID;
     default: return p.CSitBool ? Method::FISTA_OSEM : Method::OSEM;   // 2, 4, 5, 6
     }
 }
@@ -166,7 +167,8 @@ int run(const Params &p, std::ostream &log)
         b.imgs.resize(img_n * dN);
         for (int i = 0; i < dN; ++i) {
             if (!read_raw(p.d1, layer_name(p.f1, c.start + i), &b.prjs[prj_n * i], prj_n, &b.err)) return b;
-            if (m != Method::FBP && !first_image(p, &b.imgs[img_n * i], img_n, &b.err)) return b;
+            if (m != Method::FBP && m != Method::HYBRID && !first_image(p, &b.imgs[img_n * i], img_n, &b.err))
+                return b;
         }
         b.ok = true;
         return b;
@@ -203,7 +205,7 @@ int run(const Params &p, std::ostream &log)
             const int dN = b.c.end - b.c.start + 1;
             ctr_status st = CTR_OK;
             if (hipMemcpy(W.mem.prj, b.prjs.data(), sizeof(float) * prj_n * dN, hipMemcpyHostToDevice) != hipSuccess ||
-                (m != Method::FBP &&
+                (m != Method::FBP && m != Method::HYBRID &&
                  hipMemcpy(W.mem.img, b.imgs.data(), sizeof(float) * img_n * dN, hipMemcpyHostToDevice) != hipSuccess)) {
                 errors[w] = "host-to-device copy failed";
                 if (more) next.wait();
@@ -215,6 +217,9 @@ int run(const Params &p, std::ostream &log)
             case Method::AART: st = ctr_aart_execution(W.ctx, W.mem.img, W.mem.prj, dN, p.it); break;
             case Method::FISTA: st = ctr_fista_execution(W.ctx, W.mem.l2, W.mem.img, W.mem.prj, dN, p.it, 1); break;
             case Method::FBP: st = ctr_fbp_execution(W.ctx, W.mem.img, W.mem.prj, dN, 1); break;
+            case Method::HYBRID:
+                st = ctr_hybrid_execution(W.ctx, W.mem.img, W.mem.prj, dN, p.it, 1, p.baseupOrder, p.CSitBool ? 1 : 0);
+                break;
             }
             if (st == CTR_OK) st = ctr_synchronize(W.ctx);
             if (st != CTR_OK) {

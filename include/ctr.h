@@ -114,7 +114,15 @@ ctr_status ctr_aart_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
 ctr_status ctr_fbp_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
                              int dN, int prj_correction);
 
-/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is
+/* Mode 8, the FBP + OS-EM hybrid (hybrid_thread, GPU.cpp:1459-1580): FBP of
+ * the sinograms (projection correction when prj_correction != 0), a base-up of
+ * each FBP layer that maps its minimum to 0 (findMinimumX/Y and baseUp,
+ * reconstShare.cl:88-177, series order `baseup_order`, 4 in the reference),
+ * then OSEM_execution (FISTA_OSEM_execution when cs != 0) from that image. */
+ctr_status ctr_hybrid_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img, int dN, int it,
+                                int prj_correction, int baseup_order, int cs);
+
+/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is
  * 30 in the reference. */
 ctr_status ctr_power_iteration(ctr_ctx *ctx, float *L2norm, int trials);
 

@@ -52,6 +52,8 @@ int ctor_num_threads(void)
 #define RFLOOR floorf
 #define RASIN asinf
 #define RFABS fabsf
+#define REXP expf
+#define RFMIN fminf
 #include "ct_oracle_impl.h"
 #undef R
 #undef S
@@ -61,6 +63,8 @@ int ctor_num_threads(void)
 #undef RFLOOR
 #undef RASIN
 #undef RFABS
+#undef REXP
+#undef RFMIN
 
 /* ---- double (FP64 oracle) instantiation ---- */
 #define R double
@@ -71,4 +75,6 @@ int ctor_num_threads(void)
 #define RFLOOR floor
 #define RASIN asin
 #define RFABS fabs
+#define REXP exp
+#define RFMIN fmin
 #include "ct_oracle_impl.h"

@@ -78,6 +78,11 @@ void ctor_aart_execution_##S(const ctor_geom *g, const float *ang, const int *su
                     R *img, R *prj, int nz, int it, int correction_mode, float alpha);   \
 void ctor_fbp_execution_##S(const ctor_geom *g, const float *ang, R *img, R *prj,       \
                     int nz, int prj_correction, int correction_mode);                    \
+void ctor_layer_min_##S(const ctor_geom *g, const R *img, R *mins, int nz);            \
+void ctor_baseup_##S(const ctor_geom *g, R *img, const R *mins, int nz, int order);     \
+void ctor_hybrid_execution_##S(const ctor_geom *g, const float *ang, const int *sub,    \
+                    R *img, R *prj, int nz, int it, int prj_correction,                  \
+                    int correction_mode, int baseup_order, int cs, double cs_lambda);    \
 void ctor_power_iteration_##S(const ctor_geom *g, const float *ang, R *L, int trials);  \
 void ctor_fista_execution_##S(const ctor_geom *g, const float *ang, const R *L,         \
                     const int *sub, R *img, R *prj, int nz, int it, int prj_correction, \

@@ -4,7 +4,7 @@
  * Included twice by ct_oracle.c with
  *   R      the arithmetic type (float or double),
  *   S(n)   the symbol suffixer (n##_f32 / n##_f64),
- *   RSQRT/RCOS/RSIN/RFLOOR/RASIN/RFABS  the matching libm calls.
+ *   RSQRT/RCOS/RSIN/RFLOOR/RASIN/RFABS/REXP/RFMIN  the matching libm calls.
  *
  * The expressions below keep the operand order of the reference OpenCL text so
  * that the float instantiation rounds the way an FP32 OpenCL CPU device does
@@ -615,6 +615,56 @@ void S(ctor_fbp_execution)(const ctor_geom *g, const float *ang, R *img, R *prj,
     S(ctor_fbp_filter)(g, prj, fprj, nz);
     S(ctor_fbp_backprojection)(g, ang, fprj, img, nz);
     free(fprj);
+}
+
+/* findMinimumX / findMinimumY, reconstShare.cl:114-177: the minimum of each
+ * layer (fmin skips NaN); the reference's work-group loop covers the whole
+ * layer when its sizes are multiples of the work-group size. */
+void S(ctor_layer_min)(const ctor_geom *g, const R *img, R *mins, int nz)
+{
+    const size_t plane = (size_t)g->nx * g->ny;
+    for (int z = 0; z < nz; z++) {
+        R m = (R)INFINITY;
+        for (size_t i = 0; i < plane; i++) m = RFMIN(m, img[(size_t)z * plane + i]);
+        mins[z] = m;
+    }
+}
+
+/* baseUp, reconstShare.cl:88-110: x -= a * exp(g) with a the layer minimum and
+ * g the ln(1 + b) series cut at `order`, b = (x - a) / a. */
+void S(ctor_baseup)(const ctor_geom *g, R *img, const R *mins, int nz, int order)
+{
+    const size_t plane = (size_t)g->nx * g->ny;
+    #pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)(plane * nz); i++) {
+        const R a = mins[(size_t)i / plane];
+        R x = img[i];
+        const R b1 = (a != (R)0) ? (x - a) / a : (R)0;
+        R b2 = (R)1, flag = (R)-1, gg = (R)0;
+        for (int k = 1; k <= order; k++) {
+            b2 *= b1;
+            flag *= (R)-1;
+            gg += flag * b2 / (R)k;
+        }
+        x -= a * REXP(gg);
+        img[i] = x;
+    }
+}
+
+/* hybrid_thread, GPU.cpp:1459-1580 (mode 8): FBP_execution with the
+ * correction, base-up (FBPcorrection = 1), then OSEM_execution or
+ * FISTA_OSEM_execution without a second correction. */
+void S(ctor_hybrid_execution)(const ctor_geom *g, const float *ang, const int *sub, R *img, R *prj,
+                              int nz, int it, int prj_correction, int correction_mode, int baseup_order,
+                              int cs, double cs_lambda)
+{
+    R *mins = (R *)malloc(sizeof(R) * (size_t)nz);
+    S(ctor_fbp_execution)(g, ang, img, prj, nz, prj_correction, correction_mode);
+    S(ctor_layer_min)(g, img, mins, nz);
+    S(ctor_baseup)(g, img, mins, nz, baseup_order);
+    free(mins);
+    if (cs) S(ctor_fista_osem_execution)(g, ang, sub, img, prj, nz, it, 0, correction_mode, cs_lambda);
+    else S(ctor_osem_execution)(g, ang, sub, img, prj, nz, it, 0, correction_mode);
 }
 
 /* powerIteration, GPU.cpp:2079-2188: ss layers filled with 1.0, `trials` (30 in

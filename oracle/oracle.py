@@ -214,6 +214,33 @@ def fbp_execution(g, ang, prj, prj_correction=True, correction_mode=0, dtype=np.
     return img
 
 
+def layer_min(g, img, dtype=np.float32):
+    img = _arr(img, dtype)
+    mins = np.zeros(img.shape[0], dtype=dtype)
+    _call("layer_min", dtype, C.byref(g), _p(img), _p(mins), C.c_int(img.shape[0]))
+    return mins
+
+
+def baseup(g, img, mins, order, dtype=np.float32):
+    img = _arr(img, dtype).copy()
+    mins = _arr(mins, dtype)
+    _call("baseup", dtype, C.byref(g), _p(img), _p(mins), C.c_int(img.shape[0]), C.c_int(order))
+    return img
+
+
+def hybrid_execution(g, ang, prj, it, baseup_order=4, cs=False, cs_lambda=1e-4, sub=None, prj_correction=True,
+                     correction_mode=0, dtype=np.float32):
+    """Mode 8 (hybrid_thread, GPU.cpp:1459-1580); returns the image stack."""
+    prj = _arr(prj, dtype).copy()
+    nz = prj.shape[0]
+    img = np.zeros((nz, g.ny, g.nx), dtype=dtype)
+    sub = subset_order(g.ss) if sub is None else np.ascontiguousarray(sub, dtype=np.int32)
+    _call("hybrid_execution", dtype, C.byref(g), _p(_ang(ang)), _p(sub), _p(img), _p(prj), C.c_int(nz),
+          C.c_int(it), C.c_int(int(prj_correction)), C.c_int(correction_mode), C.c_int(baseup_order),
+          C.c_int(int(cs)), C.c_double(cs_lambda))
+    return img
+
+
 def power_iteration(g, ang, trials=30, dtype=np.float32):
     L = np.zeros(g.ss, dtype=dtype)
     _call("power_iteration", dtype, C.byref(g), _p(_ang(ang)), _p(L), C.c_int(trials))

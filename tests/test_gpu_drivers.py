@@ -195,3 +195,19 @@ def test_batched_graph_replays(ctx):
             assert np.array_equal(host(a), host(b)), rep
         del scratch
     ectx.close()
+
+
+@pytest.mark.parametrize("cs,corr", [(False, 0), (False, 2), (True, 0)])
+def test_hybrid_execution(ctx, cs, corr):
+    """Mode 8 (hybrid_thread, GPU.cpp:1459-1580): FBP, base-up, OS-EM."""
+    n, pa, ss, it = 64, 90, 6, 2
+    ang, g = _geom(ctx, n, pa, ss, correction_mode=corr, cs_lambda=2e-3)
+    _, sino, _ = phantom_stack(2, n, pa, 6, seed=31, i0=1e4)
+    x = dev(np.zeros((2, n, n)))
+    say("hybrid", cs, corr)
+    ctx.hybrid_execution(x, dev(sino), it, cs=cs)
+    got = host(x)
+    c32 = O.hybrid_execution(g, ang, sino, it, cs=cs, cs_lambda=2e-3, correction_mode=corr)
+    c64 = O.hybrid_execution(g, ang, sino, it, cs=cs, cs_lambda=2e-3, correction_mode=corr, dtype=np.float64)
+    e32, _ = _report(f"hybrid cs={cs} corr={corr}", got, c32, c64)
+    assert e32 < 1e-4

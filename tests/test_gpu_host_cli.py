@@ -123,3 +123,20 @@ def test_compressed_sensing_batches_overlap(tmp_path, mode):
     e = relerr(got, want)
     print(f"[parity] CLI compressed sensing mode {mode}: relerr {e:.3e}")
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("cs", [False, True])
+def test_hybrid_mode_8(tmp_path, cs):
+    nz = 3
+    extra = [("#再構成法", "8"), ("#サブセット(OSEM)", "6"), ("#反復回数", "2"), ("#ゼロパディングサイズ(FBP)", "128"),
+             ("#Base up 減少速度次数 (Hybrid)", "4")]
+    if cs:
+        extra += [("#逐次圧縮センシング計算 (FBP適応外)", "1"), ("#逐次圧縮センシング計算加算ファクター", "0.002"),
+                  ("#Number of Parallel processing per one device", "8")]
+    inp, sinos, ang = dataset(tmp_path, nz, extra)
+    run_cli(inp, f"hybrid cs={cs}")
+    g = O.geom(N, N, PA, ss=6, zp=128)
+    want = O.hybrid_execution(g, ang, sinos, 2, baseup_order=4, cs=cs, cs_lambda=0.002)
+    e = relerr(outputs(tmp_path, nz), want)
+    print(f"[parity] CLI hybrid mode 8 cs={cs}: relerr {e:.3e}")
+    assert e < 1e-4

@@ -144,7 +144,8 @@ def _tiny_dataset(tmp_path, extra):
 
 
 @pytest.mark.parametrize("extra,msg", [
-    ([("#再構成法", "8")], "method 8"),
+    ([("#再構成法", "9")], "not 1-8"),
+    ([("#再構成法", "8"), ("#ゼロパディングサイズ(FBP)", "24")], "power of two"),
     ([("#サブセット(OSEM)", "5")], "not a multiple"),
     ([("#出力画像サイズ X", "8")], "output size"),
     ([("#再構成法", "7"), ("#ゼロパディングサイズ(FBP)", "24")], "power of two"),

@@ -313,3 +313,42 @@ def test_sinogram_correction_closed_forms(mode):
     want = {0: y, 1: y * wx, 2: y * wt, 3: y * wx * wt,
             4: np.where((t * t)[None, :] < (0.5 + 0.5 * wt) ** 2, y, 0.0)}[mode]
     assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_layer_min_and_baseup():
+    """findMinimumX/Y = the NaN-skipping minimum of each layer; baseUp maps each
+    layer's minimum to exactly 0 for any order, and with a long series
+    (ln(1 + b) for |b| < 1) maps every pixel near the minimum to ~0, since
+    a * exp(ln(1 + b)) = a + (x - a) = x."""
+    rng = np.random.default_rng(3)
+    g = O.geom(16, 16, 4)
+    img = rng.uniform(-0.2, 1.0, (3, 16, 16))
+    img[1, 2, 3] = np.nan
+    mins = O.layer_min(g, img, dtype=F64)
+    assert np.array_equal(mins, np.fmin.reduce(img.reshape(3, -1), axis=1))
+    for order in (1, 4, 9):
+        out = O.baseup(g, img, mins, order, dtype=F64)
+        for z in range(3):
+            k = np.nanargmin(img[z])
+            assert out[z].ravel()[k] == 0.0
+    near = np.full((1, 16, 16), -0.5) + rng.uniform(0.0, 0.2, (1, 16, 16))   # |b| = |x - a| / |a| < 0.5
+    m = O.layer_min(g, near, dtype=F64)
+    out = O.baseup(g, near, m, 80, dtype=F64)
+    assert np.abs(out).max() < 1e-12
+
+
+def test_hybrid_is_fbp_baseup_then_osem():
+    """hybrid_thread (GPU.cpp:1459-1580): the correction is applied once, to
+    the FBP; OS-EM then starts from the based-up FBP image."""
+    from ctrecon import phantom as ph
+    img, sino, ang = ph.make_slice(32, 24, 5, 7)
+    g = O.geom(32, 32, 24, ss=4, zp=64)
+    y = sino[None].astype(np.float64)
+    got = O.hybrid_execution(g, ang, y, 2, correction_mode=2, dtype=F64)
+    yc = y.copy()
+    x = O.fbp_execution(g, ang, yc, correction_mode=2, dtype=F64)
+    x = O.baseup(g, x, O.layer_min(g, x, dtype=F64), 4, dtype=F64)
+    want = O.osem_execution(g, ang, x, O.sinogram_correction(g, ang, y, 2, dtype=F64), 2, prj_correction=False,
+                            dtype=F64)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-15)
+    assert (got >= 0).all()
