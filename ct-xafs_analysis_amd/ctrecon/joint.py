@@ -12,9 +12,8 @@ moved detector coordinates).  Each iteration:
     which also keeps x >= 1e-6), applied by the library with
     lambda = tv * step.  With precondition=True the data term is
     weighted by R = 1 / (A 1) (inverse ray lengths) and the step by
-    C = 1 / (A^T 1) (the SIRT scaling), which makes a unit step stable and
-    converges far faster than the 1 / ||A||^2 step; the TV weight is then
-    applied with the mean of C as its step;
+    C = 1 / (A^T 1) (the SIRT scaling), with which a unit step is stable;
+    the TV weight is then applied with the mean of C as its step;
   * after `warmup` iterations, one damped Gauss-Newton step on every offset:
     o[a] -= g[a] / (h[a] + mu), with g[a] the offset gradient and
     h[a] = sum over layers and bins of (d p / d X)^2 the curvature of the
@@ -22,6 +21,11 @@ moved detector coordinates).  Each iteration:
 
 Shifts and centre enter the model only through their sum, so the split is a
 convention: center = mean(o), shifts = o - center.
+
+joint_fista_tv_groups runs several stacks of different slice sizes (one
+context each) against one set of offsets, as the slices of one CT-XAFS
+dataset share the per-projection detector motion: the offset gradient and
+curvature are summed over all slices of all groups.
 """
 import math
 
@@ -47,61 +51,98 @@ def _norm2_sq(lib, nz, ny, nx, off, device, iters):
     return lam
 
 
+class _Group:
+    """One stack of equal-size slices (one library context / geometry)."""
+
+    def __init__(self, lib, y, off, tv, precondition, power_iters, x0):
+        g = lib.geom
+        self.lib, self.y = lib, y
+        nz, dev = y.shape[0], y.device
+        self.nz, self.shape = nz, (nz, g.ny, g.nx)
+        if precondition:
+            ones = torch.ones(self.shape, dtype=torch.float32, device=dev)
+            a1 = torch.empty((nz, g.pa, g.px), dtype=torch.float32, device=dev)
+            lib.align_forward(ones, off, 0.0, a1)
+            self.rw = torch.where(a1 > 1.0e-3, 1.0 / a1.clamp(min=1.0e-3), torch.zeros_like(a1))
+            at1 = torch.empty_like(ones)
+            lib.align_adjoint(torch.ones_like(a1), off, 0.0, at1)
+            self.cw = torch.where(at1 > 1.0e-3, 1.0 / at1.clamp(min=1.0e-3), torch.zeros_like(at1))
+            self.step = float(self.cw.mean())
+        else:
+            self.rw = self.cw = None
+            self.step = 1.0 / _norm2_sq(lib, nz, g.ny, g.nx, off, dev, power_iters)
+        # ISTA3D applies lambda = cs_lambda / L2norm[sub] * 0.5; this L2norm makes it tv * step
+        self.l2 = torch.tensor([0.5 * float(g.cs_lambda) / (tv * self.step)], dtype=torch.float32, device=dev)
+        self.x = (torch.zeros(self.shape, dtype=torch.float32, device=dev) if x0 is None
+                  else x0.to(dev, torch.float32).contiguous().clone())
+        self.z = self.x.clone()
+
+
+def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10, max_step=0.5, mu=1.0e-3,
+                          precondition=False, power_iters=15, chunk=0, offsets0=None, x0s=None, callback=None):
+    """Joint reconstruction of several slice stacks that share the detector
+    offsets: groups is a list of (context, y) with y [nz][pa][px] in that
+    context's geometry (sides may differ between groups; pa must not).  The
+    offset gradient and the Gauss-Newton curvature are summed over every
+    slice of every group.  Returns (xs, shifts, center, history)."""
+    pa = groups[0][0].geom.pa
+    if any(lib.geom.pa != pa for lib, _ in groups):
+        raise ValueError("all groups need the same projection count")
+    dev = groups[0][1].device
+    off = (torch.zeros(pa, dtype=torch.float32, device=dev) if offsets0 is None
+           else offsets0.to(dev, torch.float32).clone())
+    zero = torch.zeros((), dtype=torch.float32, device=dev)
+    st = [_Group(lib, y, off, tv, precondition, power_iters, None if x0s is None else x0s[i])
+          for i, (lib, y) in enumerate(groups)]
+    t = 1.0
+    hist = []
+    for k in range(iters):
+        o = off.detach().requires_grad_(align and k >= warmup)
+        h = torch.zeros(pa, dtype=torch.float32, device=dev)
+        data = 0.0
+        new_x = []
+        for G in st:
+            zr = G.z.detach().requires_grad_(True)
+            p = project(G.lib, zr, o, zero, chunk=chunk)
+            r = p - G.y
+            loss = 0.5 * (r * r).sum() if G.rw is None else 0.5 * (G.rw * r * r).sum()
+            loss.backward()        # the offset gradient accumulates over the groups
+            with torch.no_grad():
+                data += float(0.5 * (r * r).sum())
+                v = (G.z - (G.step * zr.grad if G.cw is None else G.cw * zr.grad)).contiguous()
+                x_new = torch.empty_like(v)
+                G.lib.op("ista3d", v, x_new, G.nz, 0, G.l2)
+                new_x.append(x_new)
+                if o.requires_grad:
+                    pd = p.detach()
+                    dp = torch.zeros_like(pd)
+                    dp[..., 1:-1] = 0.5 * (pd[..., 2:] - pd[..., :-2])
+                    h += (dp * dp).sum(dim=(0, 2)) if G.rw is None else (G.rw * dp * dp).sum(dim=(0, 2))
+        hist.append(data)
+        with torch.no_grad():
+            if o.requires_grad:
+                d = -o.grad / (h + mu * h.mean())
+                off += d.clamp(-max_step, max_step)
+            t_next = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
+            for G, x_new in zip(st, new_x):
+                G.z = x_new + ((t - 1.0) / t_next) * (x_new - G.x)
+                G.x = x_new
+            t = t_next
+        if callback is not None:
+            callback(k, [G.x for G in st], off, hist[-1])
+    center = off.mean()
+    return [G.x for G in st], off - center, center, hist
+
+
 def joint_fista_tv(lib, y, *, iters=100, tv=1.0e-3, align=True, warmup=10, max_step=0.5, mu=1.0e-3,
                    precondition=False, power_iters=15, chunk=0, offsets0=None, x0=None, callback=None):
     """y: measured sinograms [nz][pa][px] (float32 GPU tensor) of the context's
     geometry; x0 an optional starting image (zeros by default).  Returns
     (x, shifts, center, history) with history a list of the data term
     1/2 ||A x - y||^2 per iteration."""
-    g = lib.geom
-    nz = y.shape[0]
-    nx, ny, pa = g.nx, g.ny, g.pa
-    dev = y.device
-    off = (torch.zeros(pa, dtype=torch.float32, device=dev) if offsets0 is None
-           else offsets0.to(dev, torch.float32).clone())
-    zero = torch.zeros((), dtype=torch.float32, device=dev)
-    if precondition:
-        ones = torch.ones((nz, ny, nx), dtype=torch.float32, device=dev)
-        a1 = torch.empty((nz, g.pa, g.px), dtype=torch.float32, device=dev)
-        lib.align_forward(ones, off, 0.0, a1)
-        rw = torch.where(a1 > 1.0e-3, 1.0 / a1.clamp(min=1.0e-3), torch.zeros_like(a1))
-        at1 = torch.empty_like(ones)
-        lib.align_adjoint(torch.ones_like(a1), off, 0.0, at1)
-        cw = torch.where(at1 > 1.0e-3, 1.0 / at1.clamp(min=1.0e-3), torch.zeros_like(at1))
-        step = float(cw.mean())
-    else:
-        rw = cw = None
-        step = 1.0 / _norm2_sq(lib, nz, ny, nx, off, dev, power_iters)
-    cs = float(g.cs_lambda)
-    l2 = torch.tensor([0.5 * cs / (tv * step)], dtype=torch.float32, device=dev)   # ISTA3D: lambda = cs / l2 * 0.5
-    x = (torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev) if x0 is None
-         else x0.to(dev, torch.float32).contiguous().clone())
-    z = x.clone()
-    t = 1.0
-    hist = []
-    for k in range(iters):
-        zr = z.detach().requires_grad_(True)
-        o = off.detach().requires_grad_(align and k >= warmup)
-        p = project(lib, zr, o, zero, chunk=chunk)
-        r = p - y
-        loss = 0.5 * (r * r).sum() if rw is None else 0.5 * (rw * r * r).sum()
-        loss.backward()
-        hist.append(float(0.5 * (r.detach() * r.detach()).sum()))
-        with torch.no_grad():
-            v = (z - (step * zr.grad if cw is None else cw * zr.grad)).contiguous()
-            x_new = torch.empty_like(v)
-            lib.op("ista3d", v, x_new, nz, 0, l2)
-            if o.requires_grad:
-                pd = p.detach()
-                dp = torch.zeros_like(pd)
-                dp[..., 1:-1] = 0.5 * (pd[..., 2:] - pd[..., :-2])
-                h = (dp * dp).sum(dim=(0, 2)) if rw is None else (rw * dp * dp).sum(dim=(0, 2))
-                d = -o.grad / (h + mu * h.mean())
-                off += d.clamp(-max_step, max_step)
-            t_next = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
-            z = x_new + ((t - 1.0) / t_next) * (x_new - x)
-            x, t = x_new, t_next
-        if callback is not None:
-            callback(k, x, off, hist[-1])
-    center = off.mean()
-    return x, off - center, center, hist
+    cb = None if callback is None else (lambda k, xs, off, f: callback(k, xs[0], off, f))
+    xs, shifts, center, hist = joint_fista_tv_groups(
+        [(lib, y)], iters=iters, tv=tv, align=align, warmup=warmup, max_step=max_step, mu=mu,
+        precondition=precondition, power_iters=power_iters, chunk=chunk, offsets0=offsets0,
+        x0s=None if x0 is None else [x0], callback=cb)
+    return xs[0], shifts, center, hist

@@ -128,3 +128,46 @@ def test_joint_alignment_recovery(ctx):
     assert e1 < 0.3 and e1 < 0.2 * e0
     assert hist[-1] < 0.2 * h0[-1]
     assert float(x.min()) >= 1e-6 - 1e-12
+
+
+def test_joint_alignment_shared_offsets_uneven_stack():
+    """Slices of three sizes (two of each) from one dataset share the
+    per-projection offsets (shifts N(0, 2) px and a centre offset, Poisson
+    noise at I0 = 5e3): one offset estimate from all six slices."""
+    import torch
+    from ctrecon import Context, phantom as ph
+    from ctrecon.joint import joint_fista_tv_groups
+    pa = 360
+    rng = np.random.default_rng(12)
+    shifts = rng.normal(0, 2.0, pa)
+    center = rng.uniform(-3, 3)
+    groups, ctxs = [], []
+    for gi, n in enumerate((128, 192, 256)):
+        ys = []
+        for k in range(2):
+            _, sino, ang = ph.make_slice(n, pa, 10, 500 + gi, shifts=shifts, center=center, i0=5e3, slice_index=k)
+            ys.append(sino)
+        c = Context(0)
+        c.set_geometry(n, n, pa, 1, angles_deg=ang)
+        ctxs.append(c)
+        groups.append((c, torch.from_numpy(np.stack(ys)).cuda()))
+    th = np.deg2rad(ang.astype(np.float64))
+    B = np.stack([np.cos(th), np.sin(th)], 1)
+
+    def off_err(off):
+        e = off - (shifts + center)
+        coef, *_ = np.linalg.lstsq(B, e, rcond=None)
+        return float(np.sqrt(np.mean((e - B @ coef) ** 2)))
+
+    say("joint FISTA-TV, 3 groups x 2 slices, shared offsets")
+    xs, s, c, hist = joint_fista_tv_groups(groups, iters=100, tv=1e-3)
+    e0, e1 = off_err(np.zeros(pa)), off_err((s + c).cpu().numpy().astype(np.float64))
+    print(f"[joint] uneven stack: offset error mod translation {e0:.3f} -> {e1:.3f} px, data {hist[0]:.3e} -> "
+          f"{hist[-1]:.3e}", flush=True)
+    # the 128^2 phantoms fill the detector, so shifts of several px push them
+    # past its edge, which the model cannot represent: the error settles near
+    # 0.45 px here even without noise (0.19 px on the 256^2 case above)
+    assert e1 < 0.6 and e1 < 0.3 * e0
+    assert [tuple(x.shape) for x in xs] == [(2, 128, 128), (2, 192, 192), (2, 256, 256)]
+    for cx in ctxs:
+        cx.close()
