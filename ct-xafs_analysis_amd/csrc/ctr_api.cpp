@@ -112,6 +112,7 @@ struct ctr_ctx {
     int tw_zp = 0;
     // stack workspace
     DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull, w_min;
+    DevBuf w_rcs, w_shift;   // rotation-centre search: candidate sinograms, candidate shifts
     SliceSet set_a, set_b;
     // autotune table: n -> schedule
     std::map<int, Tune> tune;
@@ -723,6 +724,94 @@ extern "C" ctr_status ctr_hybrid_execution(ctr_ctx *c, float *img, float *prj, i
     launch_baseup(img, c->w_min.f(), dN, plane, baseup_order, c->stream);
     POST(c, "baseUp");
     return cs ? ctr_fista_osem_execution(c, img, prj, dN, it, 0) : ctr_osem_execution(c, img, prj, dN, it, 0);
+}
+
+// ===========================================================================
+// rotation-centre search (RotationCenterSearch/)
+
+namespace {
+
+// Candidate i moves the detector by start + step * i, formed in float as the
+// reference's host loop and setMask form it (rotationCenterSearch_ocl.cpp:139,
+// rotCenterSearch.cl:56).
+ctr_status upload_shifts(ctr_ctx *c, int n, float start, float step)
+{
+    std::vector<float> h((size_t)n);
+    for (int i = 0; i < n; ++i) h[i] = start + step * (float)i;
+    HIPTRY(c, c->w_shift.ensure(sizeof(float) * (size_t)n));
+    HIPTRY(c, hipMemcpyAsync(c->w_shift.p, h.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return CTR_OK;
+}
+
+// setMask's float2 angles {a*PI/180, a*PI/180 - PI_2} are formed in double
+// and stored as float, then cos() is taken in float (rotCenterSearch.cl:59-66).
+RcsAngles rcs_angles(float ang_min, float ang_max)
+{
+    const double pi = 3.14159265358979323846, pi_2 = 1.57079632679489661923;
+    RcsAngles e;
+    e.cmin = std::cos((float)((double)ang_min * pi / 180.0));
+    e.smin = std::cos((float)((double)ang_min * pi / 180.0 - pi_2));
+    e.cmax = std::cos((float)((double)ang_max * pi / 180.0));
+    e.smax = std::cos((float)((double)ang_max * pi / 180.0 - pi_2));
+    return e;
+}
+
+}  // namespace
+
+// rotCenterShift, rotCenterSearch.cl:35-48, for every candidate of the host
+// loop at rotationCenterSearch_ocl.cpp:106-147.
+extern "C" ctr_status ctr_op_rot_center_shift(ctr_ctx *c, const float *prj, float *out, int n_shift,
+                                              float shift_start, float shift_step)
+{
+    TRY(need_geom(c));
+    if (!prj || !out) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (n_shift < 1) return fail(c, CTR_INVALID_ARGUMENT, "n_shift must be >= 1 (got %d)", n_shift);
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(upload_shifts(c, n_shift, shift_start, shift_step));
+    launch_rot_shift(prj, out, c->g.pa, c->g.px, n_shift, c->w_shift.f(), c->stream);
+    POST(c, "rotCenterShift");
+    return CTR_OK;
+}
+
+// setMask + imgFocusIndex, rotCenterSearch.cl:50-92, 154-227, as
+// imgFocusIndex_estimation runs them (rotationCenterSearch_ocl.cpp:181-217).
+extern "C" ctr_status ctr_op_focus_index(ctr_ctx *c, const float *img, int n, float shift_start, float shift_step,
+                                         float ang_min, float ang_max, float *findex)
+{
+    TRY(need_geom(c));
+    if (!img || !findex) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (n < 1) return fail(c, CTR_INVALID_ARGUMENT, "n must be >= 1 (got %d)", n);
+    HIPTRY(c, hipSetDevice(c->device));
+    TRY(upload_shifts(c, n, shift_start, shift_step));
+    HIPTRY(c, c->w_partial.ensure(sizeof(float2) * l2_partial_count(n)));
+    launch_focus_index(img, n, c->g.nx, c->g.ny, c->w_shift.f(), rcs_angles(ang_min, ang_max),
+                       static_cast<float2 *>(c->w_partial.p), findex, c->stream);
+    POST(c, "imgFocusIndex");
+    return CTR_OK;
+}
+
+// rotationCenterSearch from the resliced layer on (rotationCenterSearch_ocl.cpp:
+// 489-551): shifted candidates, reconstRotCntShift_thread's g_ini fill and
+// OSEM_execution(it, prjCorrection = true) (rotationCenterSearch_ocl.cpp:
+// 281-331), then the focus indices.  The reference reconstructs the
+// candidates in batches of numParallel layers; OS-EM treats every layer on its
+// own, so one stack of all candidates gives the same images.
+extern "C" ctr_status ctr_rotation_center_search(ctr_ctx *c, const float *prj, int n_shift, float shift_start,
+                                                 float shift_step, float ang_min, float ang_max, int it,
+                                                 float init, int prj_correction, float *recon, float *findex)
+{
+    TRY(check_stack_args(c, recon, prj, n_shift));
+    if (!findex) return fail(c, CTR_INVALID_ARGUMENT, "null findex");
+    if (it < 0) return fail(c, CTR_INVALID_ARGUMENT, "it must be >= 0 (got %d)", it);
+    const ctr_geometry &g = c->g;
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, c->w_rcs.ensure(sizeof(float) * (size_t)n_shift * g.pa * g.px));
+    TRY(ctr_op_rot_center_shift(c, prj, c->w_rcs.f(), n_shift, shift_start, shift_step));
+    launch_fill(recon, (size_t)n_shift * g.nx * g.ny, init, c->stream);
+    POST(c, "fill g_ini");
+    TRY(ctr_osem_execution(c, recon, c->w_rcs.f(), n_shift, it, prj_correction));
+    return ctr_op_focus_index(c, recon, n_shift, shift_start, shift_step, ang_min, ang_max, findex);
 }
 
 // powerIteration, GPU.cpp:2079-2188: ss layers, layer s = subset s, filled

@@ -86,6 +86,15 @@ void launch_tv(int fista, const float *x, const float *v, const float *b, float 
 void launch_l2(const float *src, int nz, size_t n, float *partial, float *l2, hipStream_t stream);
 void launch_layer_min(const float *src, int nz, size_t n, float *partial, float *mins, hipStream_t stream);
 void launch_baseup(float *img, const float *mins, int nz, size_t plane, int order, hipStream_t stream);
+void launch_rot_shift(const float *prj, float *out, int pa, int px, int nshift, const float *shifts,
+                      hipStream_t stream);
+// cos(a) and cos(a - pi/2) of the two ends of the angle range (setMask's
+// cos(a_min), cos(a_max), rotCenterSearch.cl:59-66).
+struct RcsAngles {
+    float cmin, smin, cmax, smax;
+};
+void launch_focus_index(const float *img, int nz, int nx, int ny, const float *shifts, RcsAngles ends,
+                        float2 *partial, float *findex, hipStream_t stream);
 void launch_power3(const float *cnd, const float *l2, float *pimg, int nz, int nx, int ny, int ipitch,
                    hipStream_t stream);
 void launch_fbp_filter(const float *prj, float *frows, int nz, int pa, int px, int rpitch, int zp,

@@ -80,6 +80,9 @@ _SIGS = {
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
     "ctr_align_adjoint": [_P, _P, _P, _F, _P, _I],
     "ctr_align_grad_shift": [_P, _P, _P, _P, _F, _P, _I],
+    "ctr_op_rot_center_shift": [_P, _P, _P, _I, _F, _F],
+    "ctr_op_focus_index": [_P, _P, _I, _F, _F, _F, _F, _P],
+    "ctr_rotation_center_search": [_P, _P, _I, _F, _F, _F, _F, _I, _F, _I, _P, _P],
 }
 
 EXPORTED = tuple(list(_SIGS) + ["ctr_last_error", "ctr_status_string"])
@@ -142,6 +145,7 @@ class Context:
         self.h = h
         self.device = device
         self.geom = None
+        self.angles = None
         self._ext = None
         if use_torch_stream:
             self.set_stream(torch.cuda.current_stream(device).cuda_stream)
@@ -212,6 +216,7 @@ class Context:
             subp = sub.ctypes.data_as(_P)
         self._run(self.lib.ctr_set_geometry, (self.h, C.byref(g), ang.ctypes.data_as(_P), subp,), "ctr_set_geometry")
         self.geom = g
+        self.angles = ang
         return g
 
     # ---- drivers ----
@@ -233,6 +238,25 @@ class Context:
     def fbp_execution(self, img, prj, prj_correction=True):
         self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction),),
                   "ctr_fbp_execution")
+
+    def rotation_center_search(self, prj, n_shift, shift_start, shift_step, it=1, init=1.0, ang_range=None,
+                               prj_correction=True):
+        """Rotation-centre search over one layer's sinogram prj [pa, px]
+        (rotationCenterSearch_ocl.cpp:489-551).  ang_range defaults to the
+        first and last angle of the geometry.  Returns (recon [n_shift, ny, nx],
+        findex [n_shift], best shift)."""
+        t = self._torch
+        g = self.geom
+        if ang_range is None:
+            ang_range = (float(self.angles[0]), float(self.angles[-1]))
+        recon = t.empty((n_shift, g.ny, g.nx), dtype=t.float32, device=prj.device)
+        findex = t.empty((n_shift,), dtype=t.float32, device=prj.device)
+        self._run(self.lib.ctr_rotation_center_search,
+                  (self.h, _ptr(prj), n_shift, shift_start, shift_step, ang_range[0], ang_range[1], it, init,
+                   int(prj_correction), _ptr(recon), _ptr(findex)), "ctr_rotation_center_search")
+        f = findex.cpu().numpy()
+        best = int(np.argmin(f)) if np.isfinite(f).any() else 0
+        return recon, findex, np.float32(shift_start) + np.float32(shift_step) * np.float32(best)
 
     def power_iteration(self, L, trials=30):
         self._run(self.lib.ctr_power_iteration, (self.h, _ptr(L), trials,), "ctr_power_iteration")

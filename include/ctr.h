@@ -122,7 +122,7 @@ ctr_status ctr_fbp_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
 ctr_status ctr_hybrid_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img, int dN, int it,
                                 int prj_correction, int baseup_order, int cs);
 
-/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is
+/* powerIteration, GPU.cpp:2079-2188: writes L2norm[ss] (device). `trials` is
  * 30 in the reference. */
 ctr_status ctr_power_iteration(ctr_ctx *ctx, float *L2norm, int trials);
 
@@ -131,6 +131,37 @@ ctr_status ctr_power_iteration(ctr_ctx *ctx, float *L2norm, int trials);
  * ISTA3D step for any it >= 1). */
 ctr_status ctr_fista_execution(ctr_ctx *ctx, const float *L2norm, float *reconst_img,
                                float *prj_img, int dN, int it, int prj_correction);
+
+/* ---- rotation-centre search (RotationCenterSearch/) --------------------- */
+
+/* rotCenterShift, rotCenterSearch.cl:35-48, over the host loop of
+ * rotationCenterSearch_ocl.cpp:106-147: out[i] is the one-layer sinogram prj
+ * [pa][px] resampled along the detector at X + shift_start + shift_step * i
+ * (linear, border 0), for i < n_shift.  out: [n_shift][pa][px]. */
+ctr_status ctr_op_rot_center_shift(ctr_ctx *ctx, const float *prj, float *out, int n_shift,
+                                   float shift_start, float shift_step);
+
+/* setMask + imgFocusIndex, rotCenterSearch.cl:50-92, 154-227 (host:
+ * imgFocusIndex_estimation, rotationCenterSearch_ocl.cpp:181-217).  For layer
+ * i of img [n][ny][nx]: the mask of the region that candidate shift
+ * shift_start + shift_step * i keeps on the detector over the angle range
+ * [ang_min, ang_max] (degrees), and the focus index
+ * sqrt(mean over pixels whose 3 x 3 block lies in the mask of
+ * sum_8 (a_n - a)^2 / a^2).  findex: n device floats. */
+ctr_status ctr_op_focus_index(ctr_ctx *ctx, const float *img, int n, float shift_start,
+                              float shift_step, float ang_min, float ang_max, float *findex);
+
+/* rotationCenterSearch, rotationCenterSearch_ocl.cpp:489-551, from the
+ * resliced layer on: the candidate sinograms (ctr_op_rot_center_shift), each
+ * reconstructed from the constant `init` (g_ini) by OSEM_execution with `it`
+ * iterations and the projection correction when prj_correction != 0
+ * (reconstRotCntShift_thread, rotationCenterSearch_ocl.cpp:281-331), and their
+ * focus indices (ctr_op_focus_index).  recon: [n_shift][ny][nx] device output;
+ * findex: n_shift device floats.  The candidate of the smallest focus index is
+ * the estimated rotation-centre shift (rotationCenterSearch_ocl.cpp:521-549). */
+ctr_status ctr_rotation_center_search(ctr_ctx *ctx, const float *prj, int n_shift, float shift_start,
+                                      float shift_step, float ang_min, float ang_max, int it, float init,
+                                      int prj_correction, float *recon, float *findex);
 
 /* ---- single operators (device buffers, dN layers) ----------------------- */
 

@@ -92,10 +92,23 @@ void ctor_align_forward_##S(const ctor_geom *g, const float *ang, const R *img, 
 void ctor_align_adjoint_##S(const ctor_geom *g, const float *ang, const R *prj,         \
                     const R *shift, R center, R *img, int nz);                           \
 void ctor_align_grad_##S(const ctor_geom *g, const float *ang, const R *img,            \
-                    const R *gprj, const R *shift, R center, R *gshift, int nz);
+                    const R *gprj, const R *shift, R center, R *gshift, int nz);         \
+void ctor_rot_center_shift_##S(const ctor_geom *g, const R *prj, R *out, int n,        \
+                    float start, float step);                                            \
+void ctor_focus_index_##S(const ctor_geom *g, const R *img, R *findex, int n,           \
+                    float start, float step, float ang_min, float ang_max);              \
+void ctor_rotation_center_search_##S(const ctor_geom *g, const float *ang,              \
+                    const int *sub, const R *prj, int n, float start, float step,        \
+                    float ang_min, float ang_max, int it, float init, int prj_correction,\
+                    int correction_mode, R *recon, R *findex);
 
 CTOR_DECLARE(float, f32)
 CTOR_DECLARE(double, f64)
+
+/* setMask (RotationCenterSearch/rotCenterSearch.cl:50-92): mask [n][ny][nx]
+ * (0/1) of the candidates shift_i = start + step * i. */
+void ctor_rcs_mask(int nx, int ny, int n, float start, float step, float ang_min, float ang_max,
+                   unsigned char *mask);
 
 /* Number of OpenMP threads the library will use (1 when built without OpenMP). */
 int ctor_num_threads(void);

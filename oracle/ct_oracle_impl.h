@@ -826,3 +826,73 @@ void S(ctor_align_grad)(const ctor_geom *g, const float *ang, const R *img,
         gshift[a] = acc;
     }
 }
+
+/* ---- rotation-centre search (RotationCenterSearch/) ---------------------- */
+
+/* rotCenterShift, rotCenterSearch.cl:35-48, over the candidates of the host
+ * loop rotationCenterSearch_ocl.cpp:106-147: out[i] samples the one-layer
+ * sinogram linearly at (X + shift_i + 0.5, th + 0.5), shift_i = start + step*i
+ * in float (rotationCenterSearch_ocl.cpp:139). */
+void S(ctor_rot_center_shift)(const ctor_geom *g, const R *prj, R *out, int n, float start, float step)
+{
+    const int P = g->px, A = g->pa;
+    for (int i = 0; i < n; i++) {
+        const float shift = start + step * (float)i;
+        for (int th = 0; th < A; th++)
+            for (int X = 0; X < P; X++)
+                out[((size_t)i * A + th) * P + X] =
+                    S(sample_linear)(prj, P, A, ((R)X + (R)shift) + (R)0.5, (R)th + (R)0.5);
+    }
+}
+
+/* imgFocusIndex, rotCenterSearch.cl:154-227, with setMask's mask of candidate
+ * i (ctor_rcs_params): over the pixels whose 3 x 3 block lies in the mask,
+ * F = sqrt(mean of sum_8 (a_n - a)^2 / a^2).  The per-pixel terms follow the
+ * kernel's order in R; the sums over pixels are kept in double (the kernel's
+ * work-item and tree order is not reproduced). */
+void S(ctor_focus_index)(const ctor_geom *g, const R *img, R *findex, int n, float start, float step,
+                         float ang_min, float ang_max)
+{
+    const int nx = g->nx, ny = g->ny;
+    float e[4];
+    ctor_rcs_angles(ang_min, ang_max, e);
+    #pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < n; i++) {
+        const ctor_rcs_mask_t m = ctor_rcs_params(start + step * (float)i, e, nx, ny);
+        const R *p = img + (size_t)i * nx * ny;
+        double sum = 0.0, cnt = 0.0;
+        for (int Y = 0; Y < ny; Y++)
+            for (int X = 0; X < nx; X++) {
+                int all = 1;
+                for (int dy = -1; dy <= 1 && all; dy++)
+                    for (int dx = -1; dx <= 1 && all; dx++) all = ctor_rcs_in(&m, X + dx, Y + dy, nx, ny);
+                if (!all) continue;
+                const size_t k = (size_t)Y * nx + X;
+                const R a = p[k];
+                const R nb[8] = {p[k + nx], p[k - nx], p[k + 1], p[k - 1],
+                                 p[k + nx + 1], p[k - nx + 1], p[k + nx - 1], p[k - nx - 1]};
+                R sq = (R)0;
+                for (int j = 0; j < 8; j++) sq += (nb[j] - a) * (nb[j] - a);
+                sq /= a * a;
+                sum += (double)sq;
+                cnt += 1.0;
+            }
+        findex[i] = (R)sqrt(sum / cnt);
+    }
+}
+
+/* rotationCenterSearch from the resliced layer on (rotationCenterSearch_ocl.cpp:
+ * 489-551): candidates, reconstRotCntShift_thread (rotationCenterSearch_ocl.cpp:
+ * 281-331: fill with init, OSEM_execution(it, prjCorrection)), focus indices. */
+void S(ctor_rotation_center_search)(const ctor_geom *g, const float *ang, const int *sub, const R *prj, int n,
+                                    float start, float step, float ang_min, float ang_max, int it, float init,
+                                    int prj_correction, int correction_mode, R *recon, R *findex)
+{
+    const size_t np = (size_t)n * g->pa * g->px, ni = (size_t)n * g->nx * g->ny;
+    R *sh = (R *)malloc(sizeof(R) * np);
+    S(ctor_rot_center_shift)(g, prj, sh, n, start, step);
+    for (size_t i = 0; i < ni; i++) recon[i] = (R)init;
+    S(ctor_osem_execution)(g, ang, sub, recon, sh, n, it, prj_correction, correction_mode);
+    S(ctor_focus_index)(g, recon, findex, n, start, step, ang_min, ang_max);
+    free(sh);
+}

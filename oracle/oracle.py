@@ -287,3 +287,48 @@ def align_grad(g, ang, img, gprj, shift=None, center=0.0, dtype=np.float64):
     _call("align_grad", dtype, C.byref(g), _p(_ang(ang)), _p(img), _p(gprj), _p(sh), R(center), _p(out),
           C.c_int(img.shape[0]))
     return out
+
+
+# ---- rotation-centre search (RotationCenterSearch/) -------------------------
+
+def rcs_mask(nx, ny, n, start, step, ang_min, ang_max):
+    """setMask (rotCenterSearch.cl:50-92): uint8 mask [n, ny, nx]."""
+    out = np.zeros((n, ny, nx), dtype=np.uint8)
+    lib().ctor_rcs_mask(C.c_int(nx), C.c_int(ny), C.c_int(n), C.c_float(start), C.c_float(step),
+                        C.c_float(ang_min), C.c_float(ang_max), _p(out))
+    return out
+
+
+def rot_center_shift(g, prj, n, start, step, dtype=np.float32):
+    """rotCenterShift (rotCenterSearch.cl:35-48) of one layer [pa, px] for n candidates."""
+    prj = _arr(prj, dtype).reshape(g.pa, g.px)
+    out = np.zeros((n, g.pa, g.px), dtype=dtype)
+    _call("rot_center_shift", dtype, C.byref(g), _p(prj), _p(out), C.c_int(n), C.c_float(start), C.c_float(step))
+    return out
+
+
+def focus_index(g, img, start, step, ang_min, ang_max, dtype=np.float32):
+    """imgFocusIndex (rotCenterSearch.cl:154-227) of each layer of img [n, ny, nx]."""
+    img = _arr(img, dtype)
+    n = img.shape[0]
+    out = np.zeros(n, dtype=dtype)
+    _call("focus_index", dtype, C.byref(g), _p(img), _p(out), C.c_int(n), C.c_float(start), C.c_float(step),
+          C.c_float(ang_min), C.c_float(ang_max))
+    return out
+
+
+def rotation_center_search(g, ang, prj, n, start, step, it=1, init=1.0, ang_range=None, sub=None,
+                           prj_correction=True, correction_mode=0, dtype=np.float32):
+    """rotationCenterSearch from the resliced layer on
+    (rotationCenterSearch_ocl.cpp:489-551); returns (recon, findex)."""
+    prj = _arr(prj, dtype).reshape(g.pa, g.px)
+    ang = _ang(ang)
+    if ang_range is None:
+        ang_range = (float(ang[0]), float(ang[-1]))
+    sub = subset_order(g.ss) if sub is None else np.ascontiguousarray(sub, dtype=np.int32)
+    recon = np.zeros((n, g.ny, g.nx), dtype=dtype)
+    findex = np.zeros(n, dtype=dtype)
+    _call("rotation_center_search", dtype, C.byref(g), _p(ang), _p(sub), _p(prj), C.c_int(n), C.c_float(start),
+          C.c_float(step), C.c_float(ang_range[0]), C.c_float(ang_range[1]), C.c_int(it), C.c_float(init),
+          C.c_int(int(prj_correction)), C.c_int(correction_mode), _p(recon), _p(findex))
+    return recon, findex

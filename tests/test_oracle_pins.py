@@ -20,6 +20,14 @@ something independent of its own formula:
   driver quirks           FISTA_execution returns x1; AART init replaces the
                           image; FISTA_OSEM discards its first OS-EM update
   phantom                 analytic line integral of a centred disk
+  centre-shift resampling integer shifts = index shifts with zero fill, half
+                          shifts = neighbour means, np.interp elsewhere
+  centre-search mask      full 180-degree range = integer disk of radius
+                          nx/2 - |shift|; the 0-90 degree case worked by hand
+  focus index             constant image = 0, checkerboard closed form over the
+                          eroded integer disk
+  centre search           composition of its stages; the minimum of the focus
+                          index lies at the detector shift the data were made with
 """
 import numpy as np
 import pytest
@@ -352,3 +360,122 @@ def test_hybrid_is_fbp_baseup_then_osem():
                             dtype=F64)
     assert np.allclose(got, want, rtol=1e-12, atol=1e-15)
     assert (got >= 0).all()
+
+
+# ---- rotation-centre search (RotationCenterSearch/) --------------------------
+
+def _zero_fill_shift(row, k):
+    """row moved left by k samples (out[X] = row[X + k]), zeros past the ends."""
+    out = np.zeros_like(row)
+    px = row.shape[-1]
+    lo, hi = max(0, -k), min(px, px - k)
+    out[..., lo:hi] = row[..., lo + k:hi + k]
+    return out
+
+
+def test_rot_center_shift_integer_half_and_fractional():
+    rng = np.random.default_rng(4)
+    pa, px = 7, 19
+    g = O.geom(16, px, pa)
+    prj = rng.uniform(-1.0, 2.0, (pa, px)).astype(np.float32)
+    out = O.rot_center_shift(g, prj, 7, -3.0, 1.0)
+    for i in range(7):
+        assert np.array_equal(out[i], _zero_fill_shift(prj, i - 3)), i
+    half = O.rot_center_shift(g, prj, 1, 0.5, 0.0)[0]
+    assert np.array_equal(half, np.float32(0.5) * prj + np.float32(0.5) * _zero_fill_shift(prj, 1))
+    xs = np.arange(-1, px + 1, dtype=np.float64)
+    for s in (-2.3, 0.77, 4.1):
+        got = O.rot_center_shift(g, prj, 1, s, 0.0, dtype=F64)[0]
+        for th in range(pa):
+            want = np.interp(np.arange(px) + float(np.float32(s)), xs, np.concatenate([[0.0], prj[th], [0.0]]), left=0.0, right=0.0)
+            assert np.allclose(got[th], want, rtol=0, atol=1e-12), (s, th)
+
+
+@pytest.mark.parametrize("n,shift", [(40, 0.0), (40, 3.5), (40, -3.5), (33, -2.5)])
+def test_rcs_mask_full_range_is_disk(n, shift):
+    """Over 0-180 degrees the four half-planes reduce to |x| <= nx/2 - |shift|
+    and the two arcs to one circle of that radius."""
+    m = O.rcs_mask(n, n, 1, shift, 0.0, 0.0, 180.0)[0]
+    c = n // 2
+    y, x = np.mgrid[0:n, 0:n] - c
+    r = n // 2 - abs(shift)
+    assert np.array_equal(m.astype(bool), 4 * (x * x + y * y) <= int(round(4 * r * r)))
+
+
+def test_rcs_mask_quarter_range_by_hand():
+    """0-90 degrees, shift +2.5 on a 32 grid: rp = 16, rm = -13.5, the split
+    line p_div = rm cos(pi/2) is just right of x = 0; x >= 1 keeps the circle of
+    radius 16 cut by y >= rm, x <= 0 the circle of radius 13.5."""
+    n = 32
+    m = O.rcs_mask(n, n, 1, 2.5, 0.0, 0.0, 90.0)[0].astype(bool)
+    y, x = np.mgrid[0:n, 0:n] - n // 2
+    r2 = x * x + y * y
+    want = ((x >= 1) & (r2 <= 256) & (y >= -13)) | ((x <= 0) & (4 * r2 <= 729))
+    assert np.array_equal(m, want)
+
+
+def _eroded_disk(n, r2x4):
+    """Pixels whose 3 x 3 block lies in the integer disk 4(x^2+y^2) <= r2x4 and the grid."""
+    y, x = np.mgrid[0:n, 0:n] - n // 2
+    inside = 4 * (x * x + y * y) <= r2x4
+    ok = np.zeros_like(inside)
+    ok[1:-1, 1:-1] = True
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            ok[1:-1, 1:-1] &= inside[1 + dy:n - 1 + dy, 1 + dx:n - 1 + dx]
+    return ok
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_focus_index_closed_forms(dtype):
+    n = 24
+    g = O.geom(n, n, 4)
+    const = np.full((1, n, n), 0.7)
+    assert O.focus_index(g, const, 0.0, 0.0, 0.0, 180.0, dtype=dtype)[0] == 0.0
+    # checkerboard 1 +- d: 4 edge neighbours differ by 2d, 4 diagonals by 0
+    d = 0.25
+    y, x = np.mgrid[0:n, 0:n]
+    sign = np.where((x + y) % 2 == 0, 1.0, -1.0)
+    img = (1.0 + d * sign)[None]
+    for shift in (0.0, 2.5):
+        ok = _eroded_disk(n, int(round(4 * (n // 2 - shift) ** 2)))
+        a = 1.0 + d * sign[ok]
+        want = np.sqrt(np.mean(16 * d * d / (a * a)))
+        got = O.focus_index(g, img, shift, 0.0, 0.0, 180.0, dtype=dtype)[0]
+        assert abs(got - want) <= (1e-6 if dtype == F32 else 1e-14) * want, (shift, got, want)
+
+
+def _filled_phantom(n, seed):
+    """0.5 over the whole grid plus random ellipses inside the central half."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:n, 0:n] - n / 2 + 0.5
+    img = np.full((n, n), 0.5)
+    for _ in range(6):
+        cx, cy = rng.uniform(-0.25 * n, 0.25 * n, 2)
+        a, b = rng.uniform(0.03 * n, 0.12 * n, 2)
+        img += np.where(((x - cx) / a) ** 2 + ((y - cy) / b) ** 2 < 1, rng.uniform(0.2, 1.0), 0.0)
+    return img
+
+
+def test_rotation_center_search_composition_and_minimum():
+    """The search = shifted candidates, constant start, OS-EM with the
+    correction, focus index.  On data made with the rotation axis s0 bins off
+    the detector centre (a detector wide enough to lose nothing), the focus
+    index over one common mask is smallest exactly at s0; with the per-candidate
+    masks of setMask (radius nx/2 - |shift| over 180 degrees), which admit more
+    smooth background for smaller |shift|, the minimum the reference picks
+    (rotationCenterSearch_ocl.cpp:521-549) lies within one bin of s0."""
+    n, px, pa, ss = 48, 80, 72, 8
+    ang = (np.arange(pa) * 180.0 / pa).astype(np.float32)
+    g = O.geom(n, px, pa, ss=ss)
+    sino = O.align_forward(g, ang, _filled_phantom(n, 1)[None])[0]
+    for s0 in (2, -3):
+        y = _zero_fill_shift(sino, -s0).astype(np.float32)     # axis at px/2 + s0
+        rec, f = O.rotation_center_search(g, ang, y, 11, -5.0, 1.0, it=1, init=1.0, correction_mode=1)
+        cand = O.rot_center_shift(g, y, 11, -5.0, 1.0)
+        x = O.osem_execution(g, ang, np.ones((11, n, n), np.float32), cand, 1, correction_mode=1)
+        assert np.array_equal(rec, x)
+        assert np.array_equal(f, O.focus_index(g, x, -5.0, 1.0, 0.0, float(ang[-1])))
+        assert abs(int(np.argmin(f)) - 5 - s0) <= 1, (s0, f)
+        common = O.focus_index(g, x, 0.0, 0.0, 0.0, float(ang[-1]))
+        assert int(np.argmin(common)) - 5 == s0, (s0, common)
