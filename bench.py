@@ -6,6 +6,12 @@ the batched OS-EM stack).  A step is one OS-EM iteration (all 10 subsets) over
 the whole batch through ctr_osem_batched, with the sinograms and images
 already resident in HBM; value = B * steps / time.
 
+--workload reslice measures sinogram formation instead (Reslice/reslice.cl):
+A mu-t projections of an H x W camera (default the reference's 2048 x 2048
+camera and 1600 projections) resliced into H sinograms [A][W] with
+per-projection fractional x / y shifts; value = algorithmic GB/s
+(4 bytes read + 4 written per sinogram value).
+
 --workload mixed runs the uneven CT-XAFS stack instead: E energies x S slices,
 each slice's side seeded from {256, 384, 512, 768, 1024} and shared by its
 energies, attenuation with a sigmoid absorption edge over the energies.
@@ -43,6 +49,12 @@ PEAK_HBM_GBS = 8000.0
 # (profiles/k_fp_lds_k_bp_osem_1024sq_1200proj_128slices.md).
 FP_TRAFFIC_BYTES_PER_SLICE = 123471782.0
 FP_TRAFFIC_SHAPE = (1024, 1200, 10)
+# Bytes moved past L2 per k_reslice launch for 1600 projections of 2048 x 2048
+# (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled the same way; the 9 rows a
+# block reads for its 8 output rows make the fetch 9/8 of the input)
+# (profiles/k_reslice_2048sq_1600proj.md).
+RS_TRAFFIC_BYTES = 57097926552.0
+RS_TRAFFIC_SHAPE = (1600, 2048, 2048)
 
 
 def log(*a):
@@ -89,17 +101,113 @@ def cpu_baseline(n, pa, ss, ang, sino0, gpu_slice0, n_time, sino_time):
     }, rel(gpu_slice0, c32), rel(gpu_slice0, c64)
 
 
+def reslice_main(args):
+    """Sinogram formation: ctr_reslice over A projections of an n x n camera."""
+    import numpy as np
+    import torch
+
+    from ctrecon import Context
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    A, H, W = args.pa, args.n, args.n
+    dev = torch.device("cuda:0")
+    log(f"generating {A} mu-t projections of {H}x{W}")
+    g = torch.Generator(device=dev).manual_seed(2026)
+    mt = torch.rand((A, H, W), device=dev, generator=g).mul_(2.0)
+    xs = (torch.rand((A,), device=dev, generator=g) - 0.5).mul_(4.0)
+    ys = (torch.rand((A,), device=dev, generator=g) - 0.5).mul_(4.0)
+    prj = torch.empty((H, A, W), device=dev)
+    ctx = Context(0)
+    st = ctx.library_stream()
+    ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
+    ctx.synchronize()
+    # kernel time with HIP events on the library stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    nk = 3
+    e0.record(st)
+    for _ in range(nk):
+        ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
+    e1.record(st)
+    ctx.synchronize()
+    k_ms = e0.elapsed_time(e1) / nk
+    copy_ms = None
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        dst.copy_(src)
+        c1.record()
+        torch.cuda.synchronize()
+        copy_ms = c0.elapsed_time(c1)
+        log(f"reference copy of {args.copy_mib} MiB: {copy_ms:.3f} ms, "
+            f"{2 * args.copy_mib * 1048576 / (copy_ms * 1e-3) / 1e9:.0f} GB/s read + write")
+        del src, dst
+    for _ in range(args.warmup):
+        ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nbytes = 8.0 * A * H * W
+    value = nbytes * args.steps / dt / 1e9
+    achieved = nbytes / (k_ms * 1e-3) / 1e9
+    log(f"reslice: {value:.1f} GB/s ({dt / args.steps * 1e3:.2f} ms per step, kernel {k_ms:.2f} ms)")
+    out = {
+        "metric": f"reslice (sinogram formation) throughput, {H}x{W} camera, {A} projections",
+        "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: uniform random mu-t projections in [0, 2), per-projection x / y shifts in [-2, 2)",
+        "config": {"workload": f"reslice of {A} projections {H}x{W} into {H} sinograms [{A}][{W}] with fractional "
+                               f"per-projection shifts and a base-up; step = one ctr_reslice over the stack",
+                   "A": A, "H": H, "W": W},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": RS_TRAFFIC_BYTES if (A, H, W) == RS_TRAFFIC_SHAPE else None,
+                     "algorithmic_bytes": nbytes,
+                     "kernel": "k_reslice", "ms_per_launch": round(k_ms, 4)},
+    }
+    if copy_ms is not None:
+        out["device_copy"] = {"mib": args.copy_mib, "ms": round(copy_ms, 4),
+                              "gbs_read_plus_write": round(2 * args.copy_mib * 1048576 / (copy_ms * 1e-3) / 1e9, 1)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        na = 8
+        log(f"CPU baseline / oracle check on {na} projections ...")
+        mt_s = mt[:na].cpu().numpy()
+        xs_s, ys_s = xs[:na].cpu().numpy(), ys[:na].cpu().numpy()
+        t1 = time.perf_counter()
+        want = O.reslice(mt_s, xs_s, ys_s, baseup=0.1)
+        tc = time.perf_counter() - t1
+        got = prj[:, :na].cpu().numpy()
+        out["bitexact_vs_oracle"] = bool(np.array_equal(got, want))
+        out["cpu_baseline"] = {"value": round(8.0 * na * H * W / tc / 1e9, 3), "unit": "GB/s",
+                               "cores": O.num_threads(), "kind": "port",
+                               "sample": f"{na} of the {A} projections through the oracle's reslice (OpenMP)"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("uniform", "mixed"), default="uniform")
+    ap.add_argument("--workload", choices=("uniform", "mixed", "reslice"), default="uniform")
     ap.add_argument("--batch", type=int, default=128, help="uniform: slices in the batch")
-    ap.add_argument("--n", type=int, default=1024, help="uniform: slice side")
+    ap.add_argument("--n", type=int, default=None, help="uniform: slice side (1024); reslice: camera side (2048)")
     ap.add_argument("--energies", type=int, default=64, help="mixed: energies per slice")
     ap.add_argument("--slices", type=int, default=128, help="mixed: slice positions")
-    ap.add_argument("--pa", type=int, default=1200)
+    ap.add_argument("--pa", type=int, default=None, help="projections (1200; reslice: 1600)")
     ap.add_argument("--ss", type=int, default=10)
     ap.add_argument("--per-slice", type=int, default=4, help="uniform: slices timed in the per-slice-launch baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
@@ -111,6 +219,12 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box here has one GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "reslice":
+        args.n = 2048 if args.n is None else args.n
+        args.pa = 1600 if args.pa is None else args.pa
+        return reslice_main(args)
+    args.n = 1024 if args.n is None else args.n
+    args.pa = 1200 if args.pa is None else args.pa
 
     import numpy as np
     import torch

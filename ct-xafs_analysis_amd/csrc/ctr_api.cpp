@@ -113,6 +113,7 @@ struct ctr_ctx {
     // stack workspace
     DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull, w_min;
     DevBuf w_rcs, w_shift;   // rotation-centre search: candidate sinograms, candidate shifts
+    DevBuf w_xproj, w_stage; // sinogram formation: x-projections, sinograms before spike removal
     SliceSet set_a, set_b;
     // autotune table: n -> schedule
     std::map<int, Tune> tune;
@@ -724,6 +725,139 @@ extern "C" ctr_status ctr_hybrid_execution(ctr_ctx *c, float *img, float *prj, i
     launch_baseup(img, c->w_min.f(), dN, plane, baseup_order, c->stream);
     POST(c, "baseUp");
     return cs ? ctr_fista_osem_execution(c, img, prj, dN, it, 0) : ctr_osem_execution(c, img, prj, dN, it, 0);
+}
+
+// ===========================================================================
+// sinogram formation (Reslice/, the mt conversion of ImageReg share source/)
+
+namespace {
+
+ctr_status need_ctx(ctr_ctx *c)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    HIPTRY(c, hipSetDevice(c->device));
+    return CTR_OK;
+}
+
+ctr_status check_frames(ctr_ctx *c, int n, int W, int H, long long stride)
+{
+    if (n < 1 || W < 1 || H < 1) return fail(c, CTR_INVALID_ARGUMENT, "need n, W, H >= 1 (got %d, %d, %d)", n, W, H);
+    if (stride < (long long)W * H)
+        return fail(c, CTR_INVALID_ARGUMENT, "frame stride %lld is below W*H = %lld", stride, (long long)W * H);
+    return CTR_OK;
+}
+
+ctr_status check_stack(ctr_ctx *c, int A, int H, int W)
+{
+    if (A < 1 || H < 1 || W < 1) return fail(c, CTR_INVALID_ARGUMENT, "need A, H, W >= 1 (got %d, %d, %d)", A, H, W);
+    return CTR_OK;
+}
+
+}  // namespace
+
+// merge_rawhisdata, imageregistration_kernel_src.cl:868-881.
+extern "C" ctr_status ctr_merge_frames(ctr_ctx *c, const unsigned short *raw, int n, int W, int H,
+                                       long long frame_stride, float *out)
+{
+    TRY(need_ctx(c));
+    if (!raw || !out) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    TRY(check_frames(c, n, W, H, frame_stride));
+    launch_merge_frames(raw, out, n, (size_t)W * H, (size_t)frame_stride, c->stream);
+    POST(c, "merge_rawhisdata");
+    return CTR_OK;
+}
+
+// mt_conversion, imageregistration_kernel_src.cl:161-223.
+extern "C" ctr_status ctr_mt_conversion(ctr_ctx *c, const float *dark, const float *I0, const unsigned short *It,
+                                        int n, int W, int H, long long frame_stride, float *mt)
+{
+    TRY(need_ctx(c));
+    if (!dark || !I0 || !It || !mt) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    TRY(check_frames(c, n, W, H, frame_stride));
+    launch_mt_conv(dark, I0, It, mt, n, (size_t)W * H, (size_t)frame_stride, c->stream);
+    POST(c, "mt_conversion");
+    return CTR_OK;
+}
+
+// xprojection, reslice.cl:72-88, for every angle (Z_Correction's loop,
+// reslice_ocl.cpp:121-133).
+extern "C" ctr_status ctr_xprojection(ctr_ctx *c, const float *mt, int A, int H, int W, int start_x, int end_x,
+                                      float *xproj)
+{
+    TRY(need_ctx(c));
+    if (!mt || !xproj) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    TRY(check_stack(c, A, H, W));
+    launch_xproj(mt, xproj, A, H, W, start_x, end_x, c->stream);
+    POST(c, "xprojection");
+    return CTR_OK;
+}
+
+// zcorrection, reslice.cl:90-219 (Z_Correction, reslice_ocl.cpp:136-148).
+extern "C" ctr_status ctr_zcorrection(ctr_ctx *c, const float *xproj, int A, int H, int start_z, int end_z,
+                                      float *yshift)
+{
+    TRY(need_ctx(c));
+    if (!xproj || !yshift) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (A < 1 || H < 1) return fail(c, CTR_INVALID_ARGUMENT, "need A, H >= 1 (got %d, %d)", A, H);
+    if (H > ZCORR_MAX_H) return fail(c, CTR_NOT_SUPPORTED, "x-projection length %d exceeds %d", H, ZCORR_MAX_H);
+    launch_zcorr(xproj, yshift, A, H, start_z, end_z, c->stream);
+    POST(c, "zcorrection");
+    return CTR_OK;
+}
+
+// reslice, reslice.cl:52-70 (host reslice, reslice_ocl.cpp:153-208).
+extern "C" ctr_status ctr_reslice(ctr_ctx *c, const float *mt, int A, int H, int W, const float *xshift,
+                                  const float *yshift, float baseup, int correction, int y0, int n_layers,
+                                  float *prj)
+{
+    TRY(need_ctx(c));
+    if (!mt || !prj) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    TRY(check_stack(c, A, H, W));
+    if (y0 < 0 || n_layers < 1 || (long long)y0 + n_layers > H)
+        return fail(c, CTR_INVALID_ARGUMENT, "layers [%d, %d) outside [0, %d)", y0, y0 + n_layers, H);
+    launch_reslice(mt, prj, A, H, W, xshift, yshift, baseup, correction, y0, n_layers, c->stream);
+    POST(c, "reslice");
+    return CTR_OK;
+}
+
+// denoiseSinogram, reslice.cl:222-248 (host reslice_ocl.cpp:211-255).
+extern "C" ctr_status ctr_denoise_sinogram(ctr_ctx *c, const float *src, float *dst, int nz, int A, int W,
+                                           float threshold)
+{
+    TRY(need_ctx(c));
+    if (!src || !dst) return fail(c, CTR_INVALID_ARGUMENT, "null buffer");
+    if (src == dst) return fail(c, CTR_INVALID_ARGUMENT, "denoiseSinogram needs src != dst");
+    TRY(check_stack(c, A, nz, W));
+    launch_denoise(src, dst, nz, A, W, threshold, c->stream);
+    POST(c, "denoiseSinogram");
+    return CTR_OK;
+}
+
+// reslice_thread, reslice_ocl.cpp:258-350: [Z_Correction] -> reslice ->
+// [denoiseSinogram], with the x shifts at zero as the reference leaves them.
+extern "C" ctr_status ctr_reslice_execution(ctr_ctx *c, const float *mt, int A, int H, int W,
+                                            const ctr_reslice_params *p, float *yshift, float *prj)
+{
+    TRY(need_ctx(c));
+    if (!mt || !p || !yshift || !prj) return fail(c, CTR_INVALID_ARGUMENT, "null argument");
+    TRY(check_stack(c, A, H, W));
+    if (p->y0 < 0 || p->n_layers < 1 || (long long)p->y0 + p->n_layers > H)
+        return fail(c, CTR_INVALID_ARGUMENT, "layers [%d, %d) outside [0, %d)", p->y0, p->y0 + p->n_layers, H);
+    launch_fill(yshift, (size_t)A, 0.0f, c->stream);   // enqueueFillBuffer(zshift_buff, 0), reslice_ocl.cpp:293-294
+    POST(c, "fill zshift");
+    if (p->z_correction) {
+        HIPTRY(c, c->w_xproj.ensure(sizeof(float) * (size_t)A * H));
+        TRY(ctr_xprojection(c, mt, A, H, W, p->start_x, p->end_x, c->w_xproj.f()));
+        TRY(ctr_zcorrection(c, c->w_xproj.f(), A, H, p->start_z, p->end_z, yshift));
+    }
+    float *out = prj;
+    if (p->denoise) {
+        HIPTRY(c, c->w_stage.ensure(sizeof(float) * (size_t)p->n_layers * A * W));
+        out = c->w_stage.f();
+    }
+    TRY(ctr_reslice(c, mt, A, H, W, nullptr, yshift, p->baseup, 0, p->y0, p->n_layers, out));
+    if (p->denoise) TRY(ctr_denoise_sinogram(c, out, prj, p->n_layers, A, W, p->denoise_threshold));
+    return CTR_OK;
 }
 
 // ===========================================================================

@@ -88,6 +88,16 @@ void launch_layer_min(const float *src, int nz, size_t n, float *partial, float 
 void launch_baseup(float *img, const float *mins, int nz, size_t plane, int order, hipStream_t stream);
 void launch_rot_shift(const float *prj, float *out, int pa, int px, int nshift, const float *shifts,
                       hipStream_t stream);
+void launch_merge_frames(const unsigned short *raw, float *out, int n, size_t m, size_t stride, hipStream_t stream);
+void launch_mt_conv(const float *dark, const float *I0, const unsigned short *It, float *mt, int n, size_t m,
+                    size_t stride, hipStream_t stream);
+void launch_reslice(const float *mt, float *prj, int A, int H, int W, const float *xs, const float *ys,
+                    float baseup, int correction, int y0, int nyo, hipStream_t stream);
+void launch_denoise(const float *src, float *dst, int nz, int A, int W, float thr, hipStream_t stream);
+void launch_xproj(const float *mt, float *xproj, int A, int H, int W, int sx, int ex, hipStream_t stream);
+// Largest x-projection length the shift search keeps in LDS (H + 256 floats).
+constexpr int ZCORR_MAX_H = 8192;
+void launch_zcorr(const float *xproj, float *yshift, int A, int H, int sY, int eY, hipStream_t stream);
 // cos(a) and cos(a - pi/2) of the two ends of the angle range (setMask's
 // cos(a_min), cos(a_max), rotCenterSearch.cl:59-66).
 struct RcsAngles {

@@ -38,6 +38,13 @@ class Slice(C.Structure):
     _fields_ = [("n", C.c_int), ("img", C.c_void_p), ("prj", C.c_void_p)]
 
 
+class ResliceParams(C.Structure):
+    """ctr_reslice_params (include/ctr.h)."""
+    _fields_ = [("baseup", C.c_float), ("z_correction", C.c_int), ("start_x", C.c_int), ("end_x", C.c_int),
+                ("start_z", C.c_int), ("end_z", C.c_int), ("denoise", C.c_int), ("denoise_threshold", C.c_float),
+                ("y0", C.c_int), ("n_layers", C.c_int)]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -80,6 +87,13 @@ _SIGS = {
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
     "ctr_align_adjoint": [_P, _P, _P, _F, _P, _I],
     "ctr_align_grad_shift": [_P, _P, _P, _P, _F, _P, _I],
+    "ctr_merge_frames": [_P, _P, _I, _I, _I, C.c_longlong, _P],
+    "ctr_mt_conversion": [_P, _P, _P, _P, _I, _I, _I, C.c_longlong, _P],
+    "ctr_xprojection": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "ctr_zcorrection": [_P, _P, _I, _I, _I, _I, _P],
+    "ctr_reslice": [_P, _P, _I, _I, _I, _P, _P, _F, _I, _I, _I, _P],
+    "ctr_denoise_sinogram": [_P, _P, _P, _I, _I, _I, _F],
+    "ctr_reslice_execution": [_P, _P, _I, _I, _I, C.POINTER(ResliceParams), _P, _P],
     "ctr_op_rot_center_shift": [_P, _P, _P, _I, _F, _F],
     "ctr_op_focus_index": [_P, _P, _I, _F, _F, _F, _F, _P],
     "ctr_rotation_center_search": [_P, _P, _I, _F, _F, _F, _F, _I, _F, _I, _P, _P],
@@ -128,6 +142,16 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _ptr_u16(t):
+    """16-bit camera frames: a contiguous uint16 (or int16 view) GPU tensor."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype not in (torch.uint16, torch.int16) \
+            or not t.is_contiguous():
+        raise ValueError("expected a contiguous 16-bit GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+
 class Context:
     """One library context (device + stream).  Methods take float32 GPU tensors
     and run on the context stream, ordered after the work queued on torch's
@@ -159,6 +183,10 @@ class Context:
         cur = self._torch.cuda.current_stream(self.device)
         self._ext = None if s.value == cur.cuda_stream else \
             self._torch.cuda.ExternalStream(s.value, device=self.device)
+
+    def library_stream(self):
+        """The torch stream object of the stream the library launches on."""
+        return self._ext if self._ext is not None else self._torch.cuda.current_stream(self.device)
 
     def _run(self, fn, args, what):
         """Calls a library entry point in stream order with torch: the library
@@ -238,6 +266,73 @@ class Context:
     def fbp_execution(self, img, prj, prj_correction=True):
         self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction),),
                   "ctr_fbp_execution")
+
+    # ---- sinogram formation (Reslice/) ----
+    def merge_frames(self, raw, n, W, H, frame_stride=None):
+        """merge_rawhisdata: the mean [H, W] of n 16-bit frames."""
+        t = self._torch
+        out = t.empty((H, W), dtype=t.float32, device=raw.device)
+        stride = W * H if frame_stride is None else frame_stride
+        self._run(self.lib.ctr_merge_frames, (self.h, _ptr_u16(raw), n, W, H, stride, _ptr(out)), "ctr_merge_frames")
+        return out
+
+    def mt_conversion(self, dark, I0, It, n, frame_stride=None):
+        """mt_conversion: mu-t [n, H, W] of n 16-bit frames against dark / I0 [H, W]."""
+        t = self._torch
+        H, W = dark.shape
+        out = t.empty((n, H, W), dtype=t.float32, device=dark.device)
+        stride = W * H if frame_stride is None else frame_stride
+        self._run(self.lib.ctr_mt_conversion, (self.h, _ptr(dark), _ptr(I0), _ptr_u16(It), n, W, H, stride, _ptr(out)),
+                  "ctr_mt_conversion")
+        return out
+
+    def xprojection(self, mt, start_x, end_x):
+        t = self._torch
+        A, H, W = mt.shape
+        out = t.empty((A, H), dtype=t.float32, device=mt.device)
+        self._run(self.lib.ctr_xprojection, (self.h, _ptr(mt), A, H, W, start_x, end_x, _ptr(out)), "ctr_xprojection")
+        return out
+
+    def zcorrection(self, xproj, start_z, end_z, yshift=None):
+        t = self._torch
+        A, H = xproj.shape
+        ys = t.zeros((A,), dtype=t.float32, device=xproj.device) if yshift is None else yshift
+        self._run(self.lib.ctr_zcorrection, (self.h, _ptr(xproj), A, H, start_z, end_z, _ptr(ys)), "ctr_zcorrection")
+        return ys
+
+    def reslice(self, mt, xshift=None, yshift=None, baseup=0.0, correction=0, y0=0, n_layers=None, out=None):
+        """Sinograms [n_layers, A, W] (camera rows y0 ..) from projections [A, H, W]."""
+        t = self._torch
+        A, H, W = mt.shape
+        n_layers = H - y0 if n_layers is None else n_layers
+        if out is None:
+            out = t.empty((n_layers, A, W), dtype=t.float32, device=mt.device)
+        self._run(self.lib.ctr_reslice, (self.h, _ptr(mt), A, H, W, _ptr(xshift), _ptr(yshift), float(baseup),
+                                         int(correction), y0, n_layers, _ptr(out)), "ctr_reslice")
+        return out
+
+    def denoise_sinogram(self, src, threshold):
+        out = self._torch.empty_like(src)
+        nz, A, W = src.shape
+        self._run(self.lib.ctr_denoise_sinogram, (self.h, _ptr(src), _ptr(out), nz, A, W, float(threshold)),
+                  "ctr_denoise_sinogram")
+        return out
+
+    def reslice_execution(self, mt, baseup=0.0, z_correction=False, start_x=0, end_x=0, start_z=0, end_z=0,
+                          denoise=False, threshold=0.0, y0=0, n_layers=None, out=None):
+        """reslice_thread: [Z-correction] -> reslice -> [spike removal].
+        Returns (sinograms [n_layers, A, W], yshift [A])."""
+        t = self._torch
+        A, H, W = mt.shape
+        n_layers = H - y0 if n_layers is None else n_layers
+        p = ResliceParams(baseup, int(bool(z_correction)), start_x, end_x, start_z, end_z, int(bool(denoise)),
+                          threshold, y0, n_layers)
+        ys = t.empty((A,), dtype=t.float32, device=mt.device)
+        if out is None:
+            out = t.empty((n_layers, A, W), dtype=t.float32, device=mt.device)
+        self._run(self.lib.ctr_reslice_execution, (self.h, _ptr(mt), A, H, W, C.byref(p), _ptr(ys), _ptr(out)),
+                  "ctr_reslice_execution")
+        return out, ys
 
     def rotation_center_search(self, prj, n_shift, shift_start, shift_step, it=1, init=1.0, ang_range=None,
                                prj_correction=True):

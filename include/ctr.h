@@ -132,6 +132,72 @@ ctr_status ctr_power_iteration(ctr_ctx *ctx, float *L2norm, int trials);
 ctr_status ctr_fista_execution(ctr_ctx *ctx, const float *L2norm, float *reconst_img,
                                float *prj_img, int dN, int it, int prj_correction);
 
+/* ---- sinogram formation (Reslice/, mt conversion of ImageReg share source/)
+ * Projection stacks are [A][H][W] float32 (angle, camera row, camera column:
+ * the mu-t images the reference reads one per angle); the sinograms they
+ * become are [layers][A][W], the sinogram stack layout above with pa = A,
+ * px = W, so they feed the reconstruction drivers directly. */
+
+/* merge_rawhisdata, ImageReg share source/imageregistration_kernel_src.cl:
+ * 868-881: out [H][W] = mean of the n 16-bit frames raw + k * frame_stride
+ * (frame_stride = W*H + 32 for the frames of a .his file,
+ * share source/readHisRawFile.cpp:63-95). */
+ctr_status ctr_merge_frames(ctr_ctx *ctx, const unsigned short *raw, int n, int W, int H,
+                            long long frame_stride, float *out);
+
+/* mt_conversion, imageregistration_kernel_src.cl:161-223, the mu-t value (its
+ * registration mask is not formed): mt [n][H][W] = ln((I0 - dark) / (It_k -
+ * dark)) with the transmission clamped to [1e-5, 1e7], a NaN transmission read
+ * as 1 and a NaN logarithm as 0; frame k is It + k * frame_stride. */
+ctr_status ctr_mt_conversion(ctr_ctx *ctx, const float *dark, const float *I0, const unsigned short *It,
+                             int n, int W, int H, long long frame_stride, float *mt);
+
+/* xprojection, Reslice/reslice.cl:72-88 (Z_Correction, reslice_ocl.cpp:89-150):
+ * xproj [A][H] = for each angle and row, the sum over columns start_x..end_x of
+ * the linear sample at the texel corner (X, Y). */
+ctr_status ctr_xprojection(ctr_ctx *ctx, const float *mt, int A, int H, int W, int start_x, int end_x,
+                           float *xproj);
+
+/* zcorrection, reslice.cl:90-219: for each angle, 20 Levenberg-Marquardt
+ * trials (lambda 0.001) on the vertical drift yshift[A] (device, in/out) that
+ * matches the angle's x-projection profile to the middle angle's over rows
+ * start_z..end_z.  H <= 8192. */
+ctr_status ctr_zcorrection(ctr_ctx *ctx, const float *xproj, int A, int H, int start_z, int end_z,
+                           float *yshift);
+
+/* reslice, reslice.cl:52-70 (host reslice, reslice_ocl.cpp:153-208):
+ * prj [n_layers][A][W], layer l = camera row Y = y0 + l: row th, bin X is the
+ * linear sample (border 0) of projection th at (X + xshift[th] + 0.5,
+ * Y + yshift[th] + 0.5) plus baseup, times cos(asin(2|X - W/2| / W)) when
+ * correction == 1.  xshift / yshift: device [A] or NULL (zero). */
+ctr_status ctr_reslice(ctr_ctx *ctx, const float *mt, int A, int H, int W, const float *xshift,
+                       const float *yshift, float baseup, int correction, int y0, int n_layers,
+                       float *prj);
+
+/* denoiseSinogram, reslice.cl:222-248: a bin whose differences to both theta
+ * neighbours exceed `threshold` with opposite signs becomes their mean (rows
+ * past the ends clamp to the edge).  src, dst: [nz][A][W], src != dst. */
+ctr_status ctr_denoise_sinogram(ctr_ctx *ctx, const float *src, float *dst, int nz, int A, int W,
+                                float threshold);
+
+/* The reslice parameters of share source/input_parameter_reslice_class.cpp. */
+typedef struct ctr_reslice_params {
+    float baseup;            /* "#Image baseup" */
+    int z_correction;        /* "#Z-correction" == 1 */
+    int start_x, end_x;      /* "#X range of correction evaluation" */
+    int start_z, end_z;      /* "#Z range of correction evaluation" */
+    int denoise;             /* "#Noise removal" */
+    float denoise_threshold; /* "#Noise removal threshold" */
+    int y0, n_layers;        /* camera rows to reslice: y0 .. y0 + n_layers - 1 */
+} ctr_reslice_params;
+
+/* reslice_thread, reslice_ocl.cpp:258-350: yshift[A] (device out) = 0, then
+ * Z_Correction (xprojection + zcorrection) when z_correction, reslice with
+ * those vertical shifts (the x shifts stay 0, as in the reference), and
+ * denoiseSinogram when denoise.  prj: [n_layers][A][W]. */
+ctr_status ctr_reslice_execution(ctr_ctx *ctx, const float *mt, int A, int H, int W,
+                                 const ctr_reslice_params *params, float *yshift, float *prj);
+
 /* ---- rotation-centre search (RotationCenterSearch/) --------------------- */
 
 /* rotCenterShift, rotCenterSearch.cl:35-48, over the host loop of

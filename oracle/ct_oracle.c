@@ -117,6 +117,7 @@ int ctor_num_threads(void)
 #define RFABS fabsf
 #define REXP expf
 #define RFMIN fminf
+#define RLOG logf
 #include "ct_oracle_impl.h"
 #undef R
 #undef S
@@ -128,6 +129,7 @@ int ctor_num_threads(void)
 #undef RFABS
 #undef REXP
 #undef RFMIN
+#undef RLOG
 
 /* ---- double (FP64 oracle) instantiation ---- */
 #define R double
@@ -140,4 +142,5 @@ int ctor_num_threads(void)
 #define RFABS fabs
 #define REXP exp
 #define RFMIN fmin
+#define RLOG log
 #include "ct_oracle_impl.h"

@@ -100,7 +100,20 @@ void ctor_focus_index_##S(const ctor_geom *g, const R *img, R *findex, int n,   
 void ctor_rotation_center_search_##S(const ctor_geom *g, const float *ang,              \
                     const int *sub, const R *prj, int n, float start, float step,        \
                     float ang_min, float ang_max, int it, float init, int prj_correction,\
-                    int correction_mode, R *recon, R *findex);
+                    int correction_mode, R *recon, R *findex);                        \
+void ctor_merge_frames_##S(const unsigned short *raw, int n, int W, int H,              \
+                    long long stride, R *out);                                            \
+void ctor_mt_conversion_##S(const R *dark, const R *I0, const unsigned short *It,       \
+                    int n, int W, int H, long long stride, R *mt);                        \
+void ctor_xprojection_##S(const R *mt, int A, int H, int W, int sx, int ex, R *xproj);  \
+void ctor_zcorrection_##S(const R *xp, int A, int H, int sZ, int eZ, R *yshift);        \
+void ctor_reslice_##S(const R *mt, int A, int H, int W, const R *xs, const R *ys,       \
+                    float baseup, int correction, int y0, int nyo, R *prj);               \
+void ctor_denoise_sinogram_##S(const R *src, R *dst, int nz, int A, int W,              \
+                    float threshold);                                                     \
+void ctor_reslice_execution_##S(const R *mt, int A, int H, int W, float baseup,         \
+                    int z_correction, int sx, int ex, int sz, int ez, int denoise,        \
+                    float threshold, int y0, int nyo, R *yshift, R *prj);
 
 CTOR_DECLARE(float, f32)
 CTOR_DECLARE(double, f64)

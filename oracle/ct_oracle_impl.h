@@ -44,6 +44,28 @@ static inline R S(sample_linear)(const R *img, int w, int h, R u, R v)
     return ((R)1 - a) * ((R)1 - b) * t00 + a * ((R)1 - b) * t10 + ((R)1 - a) * b * t01 + a * b * t11;
 }
 
+/* CLK_FILTER_LINEAR with CLK_ADDRESS_CLAMP_TO_EDGE: texel indices clamped to
+ * the image, the weights from the unclamped coordinate. */
+static inline R S(sample_linear_edge)(const R *img, int w, int h, R u, R v)
+{
+    R uu = u - (R)0.5, vv = v - (R)0.5;
+    R fi = RFLOOR(uu), fj = RFLOOR(vv);
+    R a = uu - fi, b = vv - fj;
+    if (!(fi >= (R)-1)) fi = (R)-1;
+    if (fi > (R)w) fi = (R)w;
+    if (!(fj >= (R)-1)) fj = (R)-1;
+    if (fj > (R)h) fj = (R)h;
+    long i0 = (long)fi, j0 = (long)fj;
+    long i1 = i0 + 1, j1 = j0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 >= w ? w - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 >= w ? w - 1 : i1);
+    j0 = j0 < 0 ? 0 : (j0 >= h ? h - 1 : j0);
+    j1 = j1 < 0 ? 0 : (j1 >= h ? h - 1 : j1);
+    R t00 = img[(size_t)j0 * w + i0], t10 = img[(size_t)j0 * w + i1];
+    R t01 = img[(size_t)j1 * w + i0], t11 = img[(size_t)j1 * w + i1];
+    return ((R)1 - a) * ((R)1 - b) * t00 + a * ((R)1 - b) * t10 + ((R)1 - a) * b * t01 + a * b * t11;
+}
+
 /* angle_pr = angle[th]*PI/180.0f: float angle times a double PI, rounded to the
  * kernel's float (OSEM.cl:334, AART.cl:440, FBP.cl:180, reconstShare.cl:264). */
 static inline R S(angle_rad)(float deg)
@@ -895,4 +917,181 @@ void S(ctor_rotation_center_search)(const ctor_geom *g, const float *ang, const 
     S(ctor_osem_execution)(g, ang, sub, recon, sh, n, it, prj_correction, correction_mode);
     S(ctor_focus_index)(g, recon, findex, n, start, step, ang_min, ang_max);
     free(sh);
+}
+
+/* ---- sinogram formation (Reslice/, ImageReg share source/) --------------- */
+
+/* merge_rawhisdata, imageregistration_kernel_src.cl:868-881. */
+void S(ctor_merge_frames)(const unsigned short *raw, int n, int W, int H, long long stride, R *out)
+{
+    const size_t m = (size_t)W * H;
+    for (size_t i = 0; i < m; i++) {
+        R v = (R)0;
+        for (int k = 0; k < n; k++) v += (R)raw[i + (size_t)stride * k];
+        out[i] = v / (R)n;
+    }
+}
+
+/* mt_conversion, imageregistration_kernel_src.cl:161-223 (the mu-t value). */
+void S(ctor_mt_conversion)(const R *dark, const R *I0, const unsigned short *It, int n, int W, int H,
+                           long long stride, R *mt)
+{
+    const size_t m = (size_t)W * H;
+    for (int z = 0; z < n; z++)
+        for (size_t i = 0; i < m; i++) {
+            const R it = (R)It[i + (size_t)stride * z];
+            const R d = dark[i];
+            R tr = (I0[i] - d) / (it - d);
+            tr = (tr < (R)1.0e-5f) ? (R)1.0e-5f : tr;
+            tr = (tr > (R)1.0e7f) ? (R)1.0e7f : tr;
+            tr = isnan(tr) ? (R)1 : tr;
+            R v = RLOG(tr);
+            v = isnan(v) ? (R)0 : v;
+            mt[(size_t)z * m + i] = v;
+        }
+}
+
+/* xprojection, reslice.cl:72-88: xproj[th][Y] = sum_{j = sx..ex} of the linear
+ * sample (border 0) at the texel corner (j, Y), added in order of j to a zero
+ * start (the reference's buffer is never cleared, reslice_ocl.cpp:114). */
+void S(ctor_xprojection)(const R *mt, int A, int H, int W, int sx, int ex, R *xproj)
+{
+    #pragma omp parallel for schedule(static)
+    for (int th = 0; th < A; th++)
+        for (int Y = 0; Y < H; Y++) {
+            R acc = (R)0;
+            for (int j = sx; j <= ex; j++)
+                acc += S(sample_linear)(mt + (size_t)th * H * W, W, H, (R)j, (R)Y);
+            xproj[(size_t)th * H + Y] = acc;
+        }
+}
+
+/* reduction(), reslice.cl:35-48, over 256 work-item partial sums. */
+static R S(zc_tree)(R *loc)
+{
+    for (int s2 = 128; s2 > 0; s2 >>= 1)
+        for (int l = 0; l < s2; l++) loc[l] += loc[l + s2];
+    return loc[0];
+}
+
+/* zcorrection, reslice.cl:90-219, for a 256-item work-group: item l adds rows
+ * l, l + 256, ... in order (all H rows), then reduction().  The xprojection
+ * image is read with clamp-to-edge at texel corners (Y, angle); the target is
+ * the profile at angle A/2.  delta and nyu start at 0 and 2 (unset in the
+ * reference before the first trial, reslice.cl:111-114). */
+void S(ctor_zcorrection)(const R *xp, int A, int H, int sZ, int eZ, R *yshift)
+{
+    #pragma omp parallel for schedule(dynamic)
+    for (int ang = 0; ang < A; ang++) {
+        R *target = (R *)malloc(sizeof(R) * (size_t)H);
+        R l1[256], l2[256], l3[256];
+        const R vt = (R)(A / 2), va = (R)ang;
+        for (int Y = 0; Y < H; Y++) target[Y] = S(sample_linear_edge)(xp, H, A, (R)Y, vt);
+        R ys = yshift[ang];
+        R lambda = (R)1.0e-3f, nyu = (R)2, delta = (R)0;
+        for (int trial = 0; trial < 20; trial++) {
+            for (int l = 0; l < 256; l++) {
+                R tJJ = (R)0, tJdF = (R)0, chi2 = (R)0;
+                for (int Y = l; Y < H; Y += 256) {
+                    const R s0 = S(sample_linear_edge)(xp, H, A, (R)Y, va);
+                    const R sm1 = S(sample_linear_edge)(xp, H, A, ((R)Y + ys) - (R)1, va);
+                    const R sp1 = S(sample_linear_edge)(xp, H, A, ((R)Y + ys) + (R)1, va);
+                    const R dFdy = (sp1 - sm1) / (R)2;
+                    const R mask = (Y >= sZ && Y <= eZ) ? (R)1 : (R)0;
+                    const R dF = target[Y] - s0;
+                    tJJ += dFdy * dFdy * mask;
+                    tJdF += dFdy * dF * mask;
+                    chi2 += dF * dF * mask;
+                }
+                l1[l] = tJJ; l2[l] = tJdF; l3[l] = chi2;
+            }
+            const R tJJ = S(zc_tree)(l1), tJdF = S(zc_tree)(l2), chi2_old = S(zc_tree)(l3);
+            const R cand = ys + delta;
+            for (int l = 0; l < 256; l++) {
+                R chi2 = (R)0;
+                for (int Y = l; Y < H; Y += 256) {
+                    const R sv = S(sample_linear_edge)(xp, H, A, (R)Y + cand, va);
+                    const R mask = (Y >= sZ && Y <= eZ) ? (R)1 : (R)0;
+                    const R dF = target[Y] - sv;
+                    chi2 += dF * dF * mask;
+                }
+                l3[l] = chi2;
+            }
+            const R chi2_new = S(zc_tree)(l3);
+            delta = tJdF / tJJ / ((R)1 + lambda);
+            const R d_rho = delta * (lambda * delta + tJdF);
+            const R rho = (chi2_old - chi2_new) / d_rho;
+            const R q = (R)2 * rho - (R)1;
+            const R lA = (R)1 - q * q * q;
+            const R lB = lambda * (lA > (R)0.333f ? lA : (R)0.333f);
+            if (rho > (R)0) {
+                lambda = lB;
+                nyu = (R)2;
+                ys += delta;
+            } else {
+                lambda *= nyu;
+                nyu *= (R)2;
+            }
+        }
+        yshift[ang] = ys;
+        free(target);
+    }
+}
+
+/* reslice, reslice.cl:52-70: prj[l][th][X], camera row Y = y0 + l. */
+void S(ctor_reslice)(const R *mt, int A, int H, int W, const R *xs, const R *ys, float baseup, int correction,
+                     int y0, int nyo, R *prj)
+{
+    #pragma omp parallel for schedule(static)
+    for (int l = 0; l < nyo; l++)
+        for (int th = 0; th < A; th++)
+            for (int X = 0; X < W; X++) {
+                const int Y = y0 + l;
+                const R dx = xs ? xs[th] : (R)0, dy = ys ? ys[th] : (R)0;
+                R v = S(sample_linear)(mt + (size_t)th * H * W, W, H, ((R)X + dx) + (R)0.5, ((R)Y + dy) + (R)0.5);
+                v += (R)baseup;
+                if (correction == 1) {
+                    const R t = (R)2 * RFABS((R)X - (R)W / (R)2) / (R)W;
+                    v *= RCOS(RASIN(t));
+                }
+                prj[((size_t)l * A + th) * W + X] = v;
+            }
+}
+
+/* denoiseSinogram, reslice.cl:222-248. */
+void S(ctor_denoise_sinogram)(const R *src, R *dst, int nz, int A, int W, float threshold)
+{
+    const R thr = (R)threshold;
+    for (int z = 0; z < nz; z++)
+        for (int th = 0; th < A; th++)
+            for (int X = 0; X < W; X++) {
+                const R *b = src + (size_t)z * A * W;
+                const R f1 = b[(size_t)th * W + X];
+                const R f2 = b[(size_t)(th + 1 < A ? th + 1 : A - 1) * W + X];
+                const R f3 = b[(size_t)(th > 0 ? th - 1 : 0) * W + X];
+                const R d1 = f2 - f1, d2 = f1 - f3;
+                const R corr = (f2 + f3) / (R)2;
+                dst[((size_t)z * A + th) * W + X] = ((d1 > thr && d2 < -thr) || (d1 < -thr && d2 > thr)) ? corr : f1;
+            }
+}
+
+/* reslice_thread, reslice_ocl.cpp:258-350. */
+void S(ctor_reslice_execution)(const R *mt, int A, int H, int W, float baseup, int z_correction, int sx, int ex,
+                               int sz, int ez, int denoise, float threshold, int y0, int nyo, R *yshift, R *prj)
+{
+    for (int a = 0; a < A; a++) yshift[a] = (R)0;
+    if (z_correction) {
+        R *xp = (R *)malloc(sizeof(R) * (size_t)A * H);
+        S(ctor_xprojection)(mt, A, H, W, sx, ex, xp);
+        S(ctor_zcorrection)(xp, A, H, sz, ez, yshift);
+        free(xp);
+    }
+    if (denoise) {
+        R *tmp = (R *)malloc(sizeof(R) * (size_t)nyo * A * W);
+        S(ctor_reslice)(mt, A, H, W, NULL, yshift, baseup, 0, y0, nyo, tmp);
+        S(ctor_denoise_sinogram)(tmp, prj, nyo, A, W, threshold);
+        free(tmp);
+    } else {
+        S(ctor_reslice)(mt, A, H, W, NULL, yshift, baseup, 0, y0, nyo, prj);
+    }
 }

@@ -332,3 +332,80 @@ def rotation_center_search(g, ang, prj, n, start, step, it=1, init=1.0, ang_rang
           C.c_float(step), C.c_float(ang_range[0]), C.c_float(ang_range[1]), C.c_int(it), C.c_float(init),
           C.c_int(int(prj_correction)), C.c_int(correction_mode), _p(recon), _p(findex))
     return recon, findex
+
+
+# ---- sinogram formation (Reslice/, ImageReg share source/) ------------------
+
+def merge_frames(raw, n, W, H, stride=None, dtype=np.float32):
+    """merge_rawhisdata (imageregistration_kernel_src.cl:868-881)."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint16)
+    stride = W * H if stride is None else stride
+    out = np.zeros((H, W), dtype=dtype)
+    _call("merge_frames", dtype, _p(raw), C.c_int(n), C.c_int(W), C.c_int(H), C.c_longlong(stride), _p(out))
+    return out
+
+
+def mt_conversion(dark, I0, It, n, W, H, stride=None, dtype=np.float32):
+    """mt_conversion (imageregistration_kernel_src.cl:161-223), the mu-t value."""
+    dark, I0 = _arr(dark, dtype), _arr(I0, dtype)
+    It = np.ascontiguousarray(It, dtype=np.uint16)
+    stride = W * H if stride is None else stride
+    out = np.zeros((n, H, W), dtype=dtype)
+    _call("mt_conversion", dtype, _p(dark), _p(I0), _p(It), C.c_int(n), C.c_int(W), C.c_int(H),
+          C.c_longlong(stride), _p(out))
+    return out
+
+
+def xprojection(mt, start_x, end_x, dtype=np.float32):
+    """xprojection (reslice.cl:72-88): [A][H] from projections [A][H][W]."""
+    mt = _arr(mt, dtype)
+    A, H, W = mt.shape
+    out = np.zeros((A, H), dtype=dtype)
+    _call("xprojection", dtype, _p(mt), C.c_int(A), C.c_int(H), C.c_int(W), C.c_int(start_x), C.c_int(end_x),
+          _p(out))
+    return out
+
+
+def zcorrection(xproj, start_z, end_z, yshift=None, dtype=np.float32):
+    """zcorrection (reslice.cl:90-219): per-angle vertical drift."""
+    xproj = _arr(xproj, dtype)
+    A, H = xproj.shape
+    ys = np.zeros(A, dtype=dtype) if yshift is None else _arr(yshift, dtype).copy()
+    _call("zcorrection", dtype, _p(xproj), C.c_int(A), C.c_int(H), C.c_int(start_z), C.c_int(end_z), _p(ys))
+    return ys
+
+
+def reslice(mt, xshift=None, yshift=None, baseup=0.0, correction=0, y0=0, n_layers=None, dtype=np.float32):
+    """reslice (reslice.cl:52-70): sinograms [n_layers][A][W] from projections [A][H][W]."""
+    mt = _arr(mt, dtype)
+    A, H, W = mt.shape
+    n_layers = H - y0 if n_layers is None else n_layers
+    xs = None if xshift is None else _arr(xshift, dtype)
+    ys = None if yshift is None else _arr(yshift, dtype)
+    out = np.zeros((n_layers, A, W), dtype=dtype)
+    _call("reslice", dtype, _p(mt), C.c_int(A), C.c_int(H), C.c_int(W), _p(xs), _p(ys), C.c_float(baseup),
+          C.c_int(correction), C.c_int(y0), C.c_int(n_layers), _p(out))
+    return out
+
+
+def denoise_sinogram(src, threshold, dtype=np.float32):
+    """denoiseSinogram (reslice.cl:222-248) of sinograms [nz][A][W]."""
+    src = _arr(src, dtype)
+    nz, A, W = src.shape
+    out = np.zeros_like(src)
+    _call("denoise_sinogram", dtype, _p(src), _p(out), C.c_int(nz), C.c_int(A), C.c_int(W), C.c_float(threshold))
+    return out
+
+
+def reslice_execution(mt, baseup=0.0, z_correction=False, start_x=0, end_x=0, start_z=0, end_z=0, denoise=False,
+                      threshold=0.0, y0=0, n_layers=None, dtype=np.float32):
+    """reslice_thread (reslice_ocl.cpp:258-350); returns (sinograms, yshift)."""
+    mt = _arr(mt, dtype)
+    A, H, W = mt.shape
+    n_layers = H - y0 if n_layers is None else n_layers
+    ys = np.zeros(A, dtype=dtype)
+    out = np.zeros((n_layers, A, W), dtype=dtype)
+    _call("reslice_execution", dtype, _p(mt), C.c_int(A), C.c_int(H), C.c_int(W), C.c_float(baseup),
+          C.c_int(int(z_correction)), C.c_int(start_x), C.c_int(end_x), C.c_int(start_z), C.c_int(end_z),
+          C.c_int(int(denoise)), C.c_float(threshold), C.c_int(y0), C.c_int(n_layers), _p(ys), _p(out))
+    return out, ys

@@ -28,6 +28,17 @@ something independent of its own formula:
                           eroded integer disk
   centre search           composition of its stages; the minimum of the focus
                           index lies at the detector shift the data were made with
+  frame merge / mu-t      integer means; ln((I0 - dark) / (It - dark)) and its
+                          clamps at the three limits
+  reslice                 zero shifts = transpose (bit-exact), integer shifts =
+                          index shifts, fractional shifts = scipy's bilinear
+                          map_coordinates, cos(asin t) = sqrt(1 - t^2)
+  spike removal           crafted spikes replaced by the theta-neighbour mean,
+                          ramps and edge rows kept
+  x-projection            constant image: (ex - sx + 1) c inside, half at row 0
+  z-correction            equal profiles stay at 0 exactly; on ramp profiles the
+                          20 trials end at -2x the drift (two accepted
+                          Gauss-Newton steps, every later step raises the misfit)
 """
 import numpy as np
 import pytest
@@ -479,3 +490,118 @@ def test_rotation_center_search_composition_and_minimum():
         assert abs(int(np.argmin(f)) - 5 - s0) <= 1, (s0, f)
         common = O.focus_index(g, x, 0.0, 0.0, 0.0, float(ang[-1]))
         assert int(np.argmin(common)) - 5 == s0, (s0, common)
+
+
+# ---- sinogram formation (Reslice/, ImageReg share source/) -------------------
+
+def test_merge_frames_integer_mean():
+    rng = np.random.default_rng(2)
+    W, H, n = 5, 3, 4
+    stride = W * H + 32
+    raw = rng.integers(0, 65535, stride * n).astype(np.uint16)
+    got = O.merge_frames(raw, n, W, H, stride, dtype=F64)
+    frames = np.stack([raw[k * stride:k * stride + W * H] for k in range(n)]).astype(np.float64)
+    assert np.array_equal(got.ravel(), frames.sum(0) / n)
+
+
+def test_mt_conversion_closed_forms():
+    W, H = 4, 2
+    dark = np.full((H, W), 100.0)
+    I0 = np.full((H, W), 1100.0)
+    It = np.array([[600, 350, 100, 50], [1100, 99, 2000, 101]], np.uint16)
+    mt = O.mt_conversion(dark, I0, It, 1, W, H, dtype=F64)[0]
+    with np.errstate(divide="ignore"):
+        tr = (1100.0 - 100.0) / (It.astype(np.float64) - 100.0)
+    want = np.log(np.clip(np.where(np.isinf(tr), 1e7, tr), np.float32(1e-5), np.float32(1e7)))
+    assert np.allclose(mt, want, rtol=1e-12, atol=1e-12)
+    assert mt[0, 2] == np.log(np.float64(np.float32(1e7)))          # It = dark: +inf clamps to 1e7
+    assert mt[0, 3] == np.log(np.float64(np.float32(1e-5)))         # negative transmission clamps to 1e-5
+    zero = O.mt_conversion(np.full((1, 1), 5.0), np.full((1, 1), 5.0), np.full(1, 5, np.uint16), 1, 1, 1, dtype=F64)
+    assert zero[0, 0, 0] == 0.0                                     # 0/0 -> NaN -> 1 -> ln 1
+
+
+def test_reslice_transpose_shifts_and_bilinear():
+    from scipy.ndimage import map_coordinates
+    rng = np.random.default_rng(3)
+    A, H, W = 5, 9, 11
+    mt = rng.uniform(-1.0, 2.0, (A, H, W)).astype(np.float32)
+    plain = O.reslice(mt)
+    assert np.array_equal(plain, mt.transpose(1, 0, 2))
+    xs = np.array([0, 1, -2, 3, 0], np.float32)
+    ys = np.array([1, 0, -1, 2, -3], np.float32)
+    got = O.reslice(mt, xs, ys, baseup=0.25, y0=2, n_layers=5)
+    for th in range(A):
+        want = np.zeros((H, W), np.float32)
+        for Y in range(H):
+            for X in range(W):
+                y, x = Y + int(ys[th]), X + int(xs[th])
+                want[Y, X] = mt[th, y, x] if (0 <= y < H and 0 <= x < W) else 0.0
+        assert np.array_equal(got[:, th], want[2:7] + np.float32(0.25)), th
+    fx = rng.uniform(-2.5, 2.5, A).astype(np.float32)
+    fy = rng.uniform(-2.5, 2.5, A).astype(np.float32)
+    got = O.reslice(mt.astype(np.float64), fx, fy, dtype=F64)
+    pad = np.pad(mt.astype(np.float64), ((0, 0), (2, 2), (2, 2)))
+    Yg, Xg = np.mgrid[0:H, 0:W].astype(np.float64)
+    for th in range(A):
+        want = map_coordinates(pad[th], [Yg + float(fy[th]) + 2, Xg + float(fx[th]) + 2], order=1, mode="nearest")
+        assert np.allclose(got[:, th], want, rtol=0, atol=1e-12), th
+    corr = O.reslice(np.ones((1, 2, 8)), correction=1, dtype=F64)[0, 0]
+    t = 2 * np.abs(np.arange(8) - 4.0) / 8
+    assert np.allclose(corr, np.sqrt(1 - t * t), rtol=0, atol=1e-15)
+
+
+def test_denoise_sinogram_spikes():
+    A, W = 8, 5
+    base = np.tile(np.linspace(0.0, 1.0, A)[:, None], (1, W))       # a ramp along theta stays
+    src = base.copy()
+    src[3, 1] += 5.0                                                 # up spike
+    src[5, 4] -= 5.0                                                 # down spike
+    src[0, 2] += 5.0                                                 # first row: its theta-1 neighbour is itself
+    got = O.denoise_sinogram(src[None], 0.5, dtype=F64)[0]
+    want = src.copy()
+    want[3, 1] = (src[2, 1] + src[4, 1]) / 2
+    want[5, 4] = (src[4, 4] + src[6, 4]) / 2
+    assert np.array_equal(got, want)
+    assert np.array_equal(O.denoise_sinogram(base[None], 0.5, dtype=F64)[0], base)
+
+
+def test_xprojection_constant_image():
+    A, H, W, c = 3, 6, 10, 0.75
+    mt = np.full((A, H, W), c)
+    got = O.xprojection(mt, 2, 7, dtype=F64)
+    assert np.allclose(got[:, 1:], 6 * c, rtol=0, atol=1e-14)       # corners (j, Y) inside: 4 texels of c
+    assert np.allclose(got[:, 0], 3 * c, rtol=0, atol=1e-14)        # row -1 is the zero border
+    edge = O.xprojection(mt, 0, 0, dtype=F64)                        # column -1 is the zero border
+    assert np.allclose(edge[:, 1:], c / 2, rtol=0, atol=1e-15) and np.allclose(edge[:, 0], c / 4, atol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_zcorrection_fixed_point_and_ramp(dtype):
+    A, H = 12, 256
+    Y = np.arange(H, dtype=np.float64)
+    prof = np.exp(-((Y - 100) / 12) ** 2) + 0.1
+    assert np.array_equal(O.zcorrection(np.tile(prof, (A, 1)), 10, 240, dtype=dtype), np.zeros(A))
+    rng = np.random.default_rng(0)
+    d = rng.uniform(-2.0, 2.0, A)
+    d[A // 2 - 1:A // 2 + 1] = 0.0                                   # the target is the mean of these two angles
+    xp = np.stack([0.5 * (Y + d[a]) + 3.0 for a in range(A)])
+    e = np.array([(d[max(a - 1, 0)] + d[a]) / 2 for a in range(A)])  # each read is the mean of angles a-1, a
+    ys = O.zcorrection(xp, 20, H - 20, dtype=dtype)
+    k = np.abs(e) > 0
+    assert np.all(np.abs(ys[k] / e[k] + 2.0) < 5e-3), ys / np.where(k, e, 1)
+    assert np.all(ys[~k] == 0.0)
+
+
+def test_reslice_execution_composition():
+    rng = np.random.default_rng(5)
+    A, H, W = 6, 300, 16
+    Y = np.arange(H, dtype=np.float64)
+    drift = rng.uniform(-2.0, 2.0, A)
+    mt = np.stack([np.tile((0.5 * (Y + drift[a]) + 3.0)[:, None], (1, W)) for a in range(A)]).astype(np.float32)
+    mt[2, 150, 7] += 9.0
+    prj, ys = O.reslice_execution(mt, baseup=0.5, z_correction=True, start_x=1, end_x=W - 2, start_z=20, end_z=H - 20,
+                                  denoise=True, threshold=1.0, y0=100, n_layers=80)
+    xp = O.xprojection(mt, 1, W - 2)
+    assert np.array_equal(ys, O.zcorrection(xp, 20, H - 20))
+    want = O.denoise_sinogram(O.reslice(mt, None, ys, baseup=0.5, y0=100, n_layers=80), 1.0)
+    assert np.array_equal(prj, want)
