@@ -84,6 +84,42 @@ bool validate(const Params &p, std::string *err);
 // to `log`.  Returns 0 on success, non-zero after printing the error.
 int run(const Params &p, std::ostream &log);
 
+// ---- multi-energy driver (Reslice_CTreconst/) ------------------------------
+
+// Parameters of the multi-energy driver: the CT_reconstruction keys of the
+// same file (Reslice_CTreconst/main.cpp:13-48 defaults) and the shared keys it
+// reads (share source/input_parameter_class.cpp:318-470,
+// input_parameter_fitting_class.cpp:599-610).
+struct EnergyParams {
+    Params ct;
+    std::string input_dir;               // "#Input directory path" / -ip
+    std::string output_dir;              // "#Output directory path" / -op
+    std::string output_base;             // "#Output file base name"
+    int start_angle = -1, end_angle = -1;    // "#Angle number range" / -ar
+    int start_energy = -1, end_energy = -1;  // "#Energy number range" / -enr
+    std::vector<std::string> names;      // "#Processing parameter name" / -fpn
+    int image_x = 2048, image_y = 2048;  // "#Image pixel size"
+};
+
+// Reads both key sets of `path`; false (with a message) when the file cannot
+// be opened or a value does not parse.
+bool parse_energy_inp(const std::string &path, EnergyParams &p, std::string *err);
+
+// EnumTagString (share source/CTXAFS_analysis_share.cpp:85-96): 3-digit
+// zero-padded energy number.
+std::string energy_tag(int energy);
+
+// The projection-file prefix in `dir` (Reslice_CTreconst/main.cpp:110-131):
+// the first entry, in name order, that contains "0001.raw", without its last
+// 8 characters.
+bool find_projection_base(const std::string &dir, std::string &base, std::string *err);
+
+// Runs reslice_CTreconst_ocl (Reslice_CTreconst_ocl.cpp:26-307): for every
+// energy (or parameter name) the mu-t projections are read, resliced into
+// sinograms on the device and reconstructed there, and the layers written to
+// <output_dir>/<energy>/<output_base>NNNN.raw.  Returns 0 on success.
+int run_energies(const EnergyParams &p, std::ostream &log);
+
 }  // namespace ctrhost
 
 #endif
