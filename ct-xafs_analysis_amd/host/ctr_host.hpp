@@ -68,6 +68,18 @@ bool read_log(const std::string &path, int pa, std::vector<float> &ang, std::str
 bool read_raw(const std::string &dir, const std::string &name, float *data, size_t count, std::string *err);
 bool write_raw(const std::string &dir, const std::string &name, const float *data, size_t count,
                std::string *err);
+// readHisFile_stream (share source/readHisRawFile.cpp:63-95): frames
+// start_frame .. end_frame (1-based) of a .his file as 16-bit pixels, each
+// frame W*H pixels followed by 32 more (the next frame's header), so frame k
+// starts at out + (k - start_frame) * (W*H + 32).  The data begin 4 + c + 64
+// bytes into the file, c being the 16-bit value at byte 2.
+// With row0 / nrows only rows row0 .. row0 + nrows - 1 of each frame are
+// read, packed as [frames][nrows][W].
+bool read_his(const std::string &path, int start_frame, int end_frame, int W, int H, std::vector<unsigned short> &out,
+              std::string *err);
+bool read_his_rows(const std::string &path, int start_frame, int end_frame, int W, int H, int row0, int nrows,
+                   std::vector<unsigned short> &out, std::string *err);
+
 // fileIO.cpp:55-68: the initial image (file for fst == 1, constant ini for fst == 2).
 bool first_image(const Params &p, float *img, size_t count, std::string *err);
 

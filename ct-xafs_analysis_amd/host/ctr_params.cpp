@@ -236,6 +236,69 @@ bool write_raw(const std::string &dir, const std::string &name, const float *dat
     return true;
 }
 
+bool read_his(const std::string &path, int start_frame, int end_frame, int W, int H, std::vector<unsigned short> &out,
+              std::string *err)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
+        set_err(err, "failed to load his: " + path);
+        return false;
+    }
+    unsigned char hdr[4];
+    if (!in.read(reinterpret_cast<char *>(hdr), 4)) {
+        set_err(err, "his file " + path + " has no header");
+        return false;
+    }
+    const unsigned comment = (unsigned)hdr[2] | ((unsigned)hdr[3] << 8);   // little-endian
+    const long long frame = (long long)W * H + 32;
+    const long long skip = (long long)comment + 64 + (long long)sizeof(unsigned short) * frame * (start_frame - 1);
+    in.seekg(skip, std::ios::cur);
+    const long long n = frame * (end_frame - start_frame + 1);
+    out.assign((size_t)n, 0);
+    in.read(reinterpret_cast<char *>(out.data()), (std::streamsize)(sizeof(unsigned short) * n));
+    // the last frame needs its W*H pixels, not the 32 after them
+    if (in.gcount() < (std::streamsize)(sizeof(unsigned short) * (n - 32))) {
+        set_err(err, "his file " + path + " holds fewer than " + std::to_string(end_frame) + " frames of " +
+                         std::to_string(W) + " x " + std::to_string(H));
+        return false;
+    }
+    return true;
+}
+
+bool read_his_rows(const std::string &path, int start_frame, int end_frame, int W, int H, int row0, int nrows,
+                   std::vector<unsigned short> &out, std::string *err)
+{
+    if (row0 < 0 || nrows < 1 || row0 + nrows > H || start_frame < 1 || end_frame < start_frame) {
+        set_err(err, "bad frame or row range for " + path);
+        return false;
+    }
+    std::ifstream in(path, std::ios::binary);
+    if (!in) {
+        set_err(err, "failed to load his: " + path);
+        return false;
+    }
+    unsigned char hdr[4];
+    if (!in.read(reinterpret_cast<char *>(hdr), 4)) {
+        set_err(err, "his file " + path + " has no header");
+        return false;
+    }
+    const long long data0 = 4 + ((long long)hdr[2] | ((long long)hdr[3] << 8)) + 64;
+    const long long frame = (long long)W * H + 32;
+    const size_t per = (size_t)nrows * W;
+    out.assign(per * (size_t)(end_frame - start_frame + 1), 0);
+    for (int f = start_frame; f <= end_frame; ++f) {
+        const long long off = data0 + (long long)sizeof(unsigned short) * (frame * (f - 1) + (long long)row0 * W);
+        in.seekg(off, std::ios::beg);
+        in.read(reinterpret_cast<char *>(out.data() + per * (f - start_frame)), (std::streamsize)(sizeof(unsigned short) * per));
+        if (in.gcount() != (std::streamsize)(sizeof(unsigned short) * per)) {
+            set_err(err, "his file " + path + " holds fewer than " + std::to_string(end_frame) + " frames of " +
+                             std::to_string(W) + " x " + std::to_string(H));
+            return false;
+        }
+    }
+    return true;
+}
+
 bool first_image(const Params &p, float *img, size_t count, std::string *err)
 {
     switch (p.fst) {
