@@ -1,4 +1,4 @@
-"""Golden fixtures (tests/golden/golden_32x16.npz, written by make_golden.py):
+"""Golden fixtures (tests/golden/golden_32x16.npz and golden_around_32x16.npz, written by make_golden.py):
 the oracle reproduces them bit for bit; the GPU drivers match the FP32 values."""
 import os
 
@@ -53,3 +53,53 @@ def test_gpu_drivers_match_fixtures(ctx):
     L = dev(np.zeros(ss))
     ctx.power_iteration(L, 30)
     assert relerr(host(L), z["power_L_f32"]) < 1e-4
+
+
+FIX2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_around_32x16.npz")
+
+
+def test_oracle_reproduces_around_fixtures():
+    z = np.load(FIX2)
+    n, _, pa, ss, zp = (int(v) for v in z["geom"])
+    g = O.geom(n, n, pa, ss=ss, zp=zp)
+    W, H, stride = 32, 6, int(z["frame_stride"][0])
+    assert np.array_equal(O.hybrid_execution(g, z["angles"], z["sino"], 2, baseup_order=4), z["hybrid_it2_f32"])
+    dm = O.merge_frames(z["dark"], 3, W, H, stride)
+    im = O.merge_frames(z["i0"], 3, W, H, stride)
+    assert np.array_equal(O.mt_conversion(dm, im, z["it"], pa, W, H, stride), z["mt_f32"])
+    assert np.array_equal(O.reslice(z["mt_f32"], z["xshift"], z["yshift"], baseup=0.25, y0=1, n_layers=4),
+                          z["reslice_f32"])
+    assert np.array_equal(O.denoise_sinogram(z["spiky"], 0.5), z["denoise_f32"])
+    assert np.array_equal(O.zcorrection(z["xproj_in"], 20, 230), z["zcorr_f32"])
+    assert np.array_equal(O.xprojection(z["mt_f32"], 2, 29), z["xprojection_f32"])
+    rec, f = O.rotation_center_search(g, z["angles"], z["sino"][0], 7, -1.5, 0.5, it=1, init=1.0, correction_mode=1)
+    assert np.array_equal(rec, z["rcs_recon_f32"]) and np.array_equal(f, z["rcs_findex_f32"])
+
+
+@pytest.mark.gpu
+def test_gpu_around_fixtures(ctx):
+    import torch
+    z = np.load(FIX2)
+    n, _, pa, ss, zp = (int(v) for v in z["geom"])
+    ang = z["angles"]
+    W, H, stride = 32, 6, int(z["frame_stride"][0])
+    u16 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.uint16).view(np.int16)).cuda()
+    say("golden around: hybrid")
+    ctx.set_geometry(n, n, pa, ss, zp=zp, angles_deg=ang)
+    x = dev(np.zeros((1, n, n)))
+    ctx.hybrid_execution(x, dev(z["sino"]), 2)
+    assert relerr(host(x), z["hybrid_it2_f32"]) < 1e-4
+    say("golden around: frames, mu-t, reslice, spikes, x-projection, z-correction")
+    dm = ctx.merge_frames(u16(z["dark"]), 3, W, H, stride)
+    im = ctx.merge_frames(u16(z["i0"]), 3, W, H, stride)
+    mt = ctx.mt_conversion(dm, im, u16(z["it"]), pa, stride)
+    assert np.allclose(host(mt), z["mt_f32"], rtol=2e-6, atol=2e-6)
+    r = ctx.reslice(dev(z["mt_f32"]), dev(z["xshift"]), dev(z["yshift"]), baseup=0.25, y0=1, n_layers=4)
+    assert np.array_equal(host(r), z["reslice_f32"])
+    assert np.array_equal(host(ctx.denoise_sinogram(dev(z["spiky"]), 0.5)), z["denoise_f32"])
+    assert np.array_equal(host(ctx.zcorrection(dev(z["xproj_in"]), 20, 230)), z["zcorr_f32"])
+    assert relerr(host(ctx.xprojection(dev(z["mt_f32"]), 2, 29)), z["xprojection_f32"]) < 1e-6
+    say("golden around: centre search")
+    ctx.set_geometry(n, n, pa, ss, correction_mode=1, angles_deg=ang)
+    rec, f, _ = ctx.rotation_center_search(dev(z["sino"][0]), 7, -1.5, 0.5, it=1, init=1.0)
+    assert relerr(host(rec), z["rcs_recon_f32"]) < 1e-5 and relerr(host(f), z["rcs_findex_f32"]) < 1e-4
