@@ -1381,7 +1381,7 @@ extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, int fp_rays, int fp_wa
     if (!c) return CTR_INVALID_ARGUMENT;
     const bool ok = (fp_rays == 0 || fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
                     (fp_waves == 2 || fp_waves == 4) &&
-                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8) && n >= 2 &&
+                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8 || bp_rows == 16) && n >= 2 &&
                     !(fp_rays == 0 && n > LDS_FP_MAX_SIDE);
     if (!ok)
         return fail(c, CTR_INVALID_ARGUMENT, "schedule (n=%d, rays=%d, waves=%d, rows=%d) not supported", n, fp_rays,
@@ -1452,7 +1452,7 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
             if (ms < best) { best = ms; best_t.fp_wpb = w; best_t.fp_rays = rays; }
         }
     best = 1e30;
-    const int ropts[3] = {2, 4, 8};
+    const int ropts[4] = {2, 4, 8, 16};
     for (int r : ropts) {
         S.fp_wpb = best_t.fp_wpb;
         S.fp_rays = best_t.fp_rays;

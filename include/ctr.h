@@ -292,7 +292,7 @@ ctr_status ctr_autotune(ctr_ctx *ctx, int n, int *fp_rays, int *fp_waves, int *b
 
 /* Sets the schedule for slice side n explicitly (the values ctr_autotune
  * would choose): fp_rays in {0, 64, 32, 16, 8}, fp_waves in {2, 4},
- * bp_rows in {1, 2, 4, 8}. */
+ * bp_rows in {1, 2, 4, 8, 16}. */
 ctr_status ctr_set_schedule(ctr_ctx *ctx, int n, int fp_rays, int fp_waves, int bp_rows);
 
 /* Average duration (ms) of the last timed forward-projection and

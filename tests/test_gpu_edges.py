@@ -26,7 +26,7 @@ def test_every_projector_schedule(ctx, rays, waves):
     ang, g = _geom(ctx, n, n, pa, ss)
     rng = np.random.default_rng(5)
     img = rng.uniform(0.0, 1.0, (nz, n, n)).astype(np.float32)
-    for bp_rows in (1, 2, 4, 8):
+    for bp_rows in (1, 2, 4, 8, 16):
         ctx.set_schedule(n, rays, waves, bp_rows)
         say("schedule", rays, waves, bp_rows)
         out = dev(np.zeros((nz, pa, n)))
