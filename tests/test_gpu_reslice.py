@@ -53,6 +53,14 @@ def test_reslice_shifts_baseup_correction(ctx):
     got = host(ctx.reslice(dev(mt), None, dev(ys), correction=1))
     want = O.reslice(mt, None, ys, correction=1)
     assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
+    # shifts that move whole rows and columns past the image and its border
+    # band (the row-by-row path and the all-border path of the kernel)
+    far_y = np.where(np.arange(A) % 2 == 0, -(H // 2 + 7.25), H // 2 + 6.5).astype(np.float32)
+    far_x = np.where(np.arange(A) % 3 == 0, W + 9.5, 0.75).astype(np.float32)
+    say("reslice far shifts")
+    got = host(ctx.reslice(dev(mt), dev(far_x), dev(far_y), baseup=-0.5, y0=3, n_layers=H - 3))
+    want = O.reslice(mt, far_x, far_y, baseup=-0.5, y0=3, n_layers=H - 3)
+    assert np.array_equal(got, want), np.abs(got - want).max()
 
 
 def test_denoise_and_xprojection(ctx):
@@ -78,6 +86,22 @@ def _drifted_ramps(A, H, W, seed):
     drift = rng.uniform(-2.0, 2.0, A)
     prof = [0.5 * (Y + drift[a]) + 3.0 + 0.2 * np.sin((Y + drift[a]) / 9.0) for a in range(A)]
     return np.stack([np.tile(p[:, None], (1, W)) for p in prof]).astype(np.float32)
+
+
+@pytest.mark.parametrize("H,sz,ez", [(300, 10, 290), (64, 0, 63), (1000, 200, 700)])
+def test_zcorrection_row_counts(ctx, H, sz, ez):
+    """Profiles whose length is not a multiple of the 256-item group, and
+    evaluation ranges touching the ends."""
+    A = 12
+    rng = np.random.default_rng(H)
+    Y = np.arange(H, dtype=np.float64)
+    xp = np.stack([0.5 * (Y + d) + 2.0 + 0.3 * np.sin((Y + d) / 7.0) for d in rng.uniform(-1.5, 1.5, A)])
+    xp = xp.astype(np.float32)
+    say("zcorrection", H, sz, ez)
+    got = host(ctx.zcorrection(dev(xp), sz, ez))
+    want = O.zcorrection(xp, sz, ez)
+    print(f"[parity] zcorrection H {H}: max |diff| {np.abs(got - want).max():.3e} px", flush=True)
+    assert np.abs(got - want).max() < 1e-4
 
 
 def test_zcorrection_and_reslice_execution(ctx):
