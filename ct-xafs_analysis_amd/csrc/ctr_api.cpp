@@ -377,9 +377,9 @@ ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer,
 }
 
 ctr_status run_bp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer, float alpha, const float *L,
-                  int to_aux)
+                  int to_aux, int write_t = 0)
 {
-    BpParams p{sub, c->g.ss, c->g.pa / c->g.ss, c->g.pa, sub_from_layer, alpha, L, to_aux, 0};
+    BpParams p{sub, c->g.ss, c->g.pa / c->g.ss, c->g.pa, sub_from_layer, alpha, L, to_aux, 0, write_t};
     time_begin(c, 1);
     launch_bp(var, S.bp_rows, launch_of(c, S, G_BP), p, c->stream);
     POST(c, "backprojection");
@@ -595,11 +595,12 @@ extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int
     SliceSet &S = c->set_a;
     TRY(build_stack(c, S, dN, b));
     TRY(run_kind(c, S, G_PAD));
+    TRY(run_kind(c, S, G_TR));
+    // each update also refreshes the transposed copy the next projection reads
     for (int i = 0; i < it; ++i)
         for (int k = 0; k < g.ss; ++k) {
-            TRY(run_kind(c, S, G_TR));
             TRY(run_fp(c, S, FP_RATIO, c->sub[k], 0, 0));
-            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0));
+            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
         }
     TRY(run_kind(c, S, G_EW));
     return CTR_OK;
@@ -1254,14 +1255,14 @@ bool same_key(const std::vector<ctr_slice> &a, const ctr_slice *b, int n)
     return true;
 }
 
-// One OS-EM iteration over the whole plan: per subset a transpose, a
-// projection and a backprojection launch, each covering every slice.
+// One OS-EM iteration over the whole plan: per subset a projection and a
+// backprojection launch, each covering every slice; the backprojection also
+// writes the transposed copy the next projection reads.
 ctr_status batched_iteration(ctr_ctx *c, BatchPlan &P)
 {
     for (int k = 0; k < c->g.ss; ++k) {
-        TRY(run_kind(c, P.set, G_TR));
         TRY(run_fp(c, P.set, FP_RATIO, c->sub[k], 0, 0));
-        TRY(run_bp(c, P.set, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0));
+        TRY(run_bp(c, P.set, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
     }
     return CTR_OK;
 }
@@ -1329,6 +1330,7 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
     }
     BatchPlan &P = *c->plan;
     TRY(run_kind(c, P.set, G_PAD));
+    TRY(run_kind(c, P.set, G_TR));
     const bool graph = use_graph && !c->sync_each && !c->timing;
     if (graph && it > 0 && !P.exec) {
         HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

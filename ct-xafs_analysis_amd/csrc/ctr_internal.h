@@ -68,6 +68,7 @@ struct BpParams {
     const float *L;       // L2norm (FISTA backprojection) or null
     int to_aux;           // write the result to aux instead of the padded image
     int nblk;             // logical blocks of the launch (set by launch_bp)
+    int write_t;          // also write the transposed padded copy (the next projection reads it)
 };
 
 // ---- launchers (ctr_kernels.hip) ----
