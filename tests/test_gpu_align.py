@@ -103,3 +103,44 @@ def test_shift_gradient_vs_fp64_and_finite_differences(ctx):
     print(f"[fd] shift grad relerr vs FD {e_fd:.2e}; centre grad {got.sum():.6e} vs FD {fdc:.6e}", flush=True)
     assert e_fd < 1e-3
     assert abs(got.sum() - fdc) <= 1e-3 * max(1.0, abs(fdc))
+
+
+@pytest.mark.parametrize("nx,ny,px,pa", [(96, 64, 128, 60), (3600, 40, 3600, 6)])
+def test_nonsquare_and_gather_fallback(ctx, nx, ny, px, pa):
+    """nx != ny with a detector wider than the image (the LDS-tiled forward
+    and gradient), and a side past 56 x 64 = 3584 samples per ray (the direct
+    gather kernels): forward, adjoint and shift gradient against FP64."""
+    rng = np.random.default_rng(nx + ny)
+    ang = (np.arange(pa) * (180.0 / pa) + 0.37).astype(np.float32)
+    ctx.set_geometry(nx, px, pa, 1, ny=ny, angles_deg=ang)
+    g = O.geom(nx, px, pa, ss=1, ny=ny)
+    # a smooth image that falls to 0 at its edges: the shift gradient of
+    # bilinear sampling jumps where a sample crosses a texel edge, by the
+    # image's second difference there (the zero border included), and FP32 and
+    # FP64 coordinates put a few samples on either side of an edge
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    img = (np.sin(np.pi * (xx + 0.5) / nx) ** 2 * np.sin(np.pi * (yy + 0.5) / ny) ** 2
+           * (1.0 + 0.3 * np.sin(xx / 7.0 + 0.3) * np.cos(yy / 5.0)))[None].astype(np.float32)
+    shifts = rng.normal(0, 1.5, pa).astype(np.float32)
+    gp = rng.normal(0, 1, (1, pa, px)).astype(np.float32)
+    say("align ops", nx, ny, px, pa)
+    # FP32 coordinates near 1800 carry 1.2e-4 px of rounding, so at large
+    # sides the bound is the FP32 oracle's own distance from FP64
+    f64 = O.align_forward(g, ang, img, shifts, 0.4)
+    a64 = O.align_adjoint(g, ang, gp, shifts, 0.4)
+    g64 = O.align_grad(g, ang, img, gp, shifts, 0.4)
+    b_f = max(1e-5, 2.0 * relerr(O.align_forward(g, ang, img, shifts, 0.4, dtype=np.float32), f64))
+    b_a = max(1e-5, 2.0 * relerr(O.align_adjoint(g, ang, gp, shifts, 0.4, dtype=np.float32), a64))
+    b_g = max(1e-4, 2.0 * relerr(O.align_grad(g, ang, img, gp, shifts, 0.4, dtype=np.float32), g64))
+    out = dev(np.zeros((1, pa, px)))
+    ctx.align_forward(dev(img), dev(shifts), 0.4, out)
+    e_f = relerr(host(out), f64)
+    adj = dev(np.zeros((1, ny, nx)))
+    ctx.align_adjoint(dev(gp), dev(shifts), 0.4, adj)
+    e_a = relerr(host(adj), a64)
+    gs = dev(np.zeros(pa))
+    ctx.align_grad_shift(dev(img), dev(gp), dev(shifts), 0.4, gs)
+    e_g = relerr(host(gs), g64)
+    print(f"[parity] align {nx}x{ny}, {px} bins, {pa} angles: forward {e_f:.2e} (bound {b_f:.1e}), adjoint "
+          f"{e_a:.2e} ({b_a:.1e}), shift gradient {e_g:.2e} ({b_g:.1e})", flush=True)
+    assert e_f < b_f and e_a < b_a and e_g < b_g
