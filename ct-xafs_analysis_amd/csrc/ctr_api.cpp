@@ -114,6 +114,7 @@ struct ctr_ctx {
     DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull, w_min;
     DevBuf w_rcs, w_shift;   // rotation-centre search: candidate sinograms, candidate shifts
     DevBuf w_xproj, w_stage; // sinogram formation: x-projections, sinograms before spike removal
+    DevBuf w_gpad;           // alignment adjoint: projection rows with zero margins
     SliceSet set_a, set_b;
     // autotune table: n -> schedule
     std::map<int, Tune> tune;
@@ -1517,8 +1518,9 @@ extern "C" ctr_status ctr_align_adjoint(ctr_ctx *c, const float *prj, const floa
     TRY(op_prologue(c, prj, img, dN));
     if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
     const ctr_geometry &g = c->g;
-    launch_align_adjoint(prj, static_cast<const float2 *>(c->cs.p), shifts, center, img, dN, g.nx, g.ny, g.pa, g.px,
-                         c->stream);
+    HIPTRY(c, c->w_gpad.ensure(sizeof(float) * (size_t)dN * g.pa * align_adj_pitch(g.px)));
+    launch_align_adjoint(prj, c->w_gpad.f(), static_cast<const float2 *>(c->cs.p), shifts, center, img, dN, g.nx,
+                         g.ny, g.pa, g.px, c->stream);
     POST(c, "align adjoint");
     return CTR_OK;
 }
