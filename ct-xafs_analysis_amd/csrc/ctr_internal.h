@@ -23,9 +23,9 @@
 namespace ctr {
 
 constexpr int PAD = 100;
-// Largest max(nx, ny) the LDS-tiled projector takes (one tile of 56 samples
+// Largest max(nx, ny) the LDS-tiled projector takes (one tile of 60 samples
 // per lane of a wave); larger slices use the direct-gather projector.
-constexpr int LDS_FP_MAX_SIDE = 56 * 64;
+constexpr int LDS_FP_MAX_SIDE = 60 * 64;
 
 struct SliceDev {
     float *img;          // padded image (pitch ipitch, ny + 2 PAD rows)

@@ -105,10 +105,10 @@ def test_shift_gradient_vs_fp64_and_finite_differences(ctx):
     assert abs(got.sum() - fdc) <= 1e-3 * max(1.0, abs(fdc))
 
 
-@pytest.mark.parametrize("nx,ny,px,pa", [(96, 64, 128, 60), (3600, 40, 3600, 6)])
+@pytest.mark.parametrize("nx,ny,px,pa", [(96, 64, 128, 60), (3900, 40, 3900, 6)])
 def test_nonsquare_and_gather_fallback(ctx, nx, ny, px, pa):
     """nx != ny with a detector wider than the image (the LDS-tiled forward
-    and gradient), and a side past 56 x 64 = 3584 samples per ray (the direct
+    and gradient), and a side past 60 x 64 = 3840 samples per ray (the direct
     gather kernels): forward, adjoint and shift gradient against FP64."""
     rng = np.random.default_rng(nx + ny)
     ang = (np.arange(pa) * (180.0 / pa) + 0.37).astype(np.float32)

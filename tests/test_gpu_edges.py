@@ -66,19 +66,19 @@ def test_odd_and_rectangular_geometries(ctx, nx, ny, px, pa, ss):
 
 
 def test_lds_projector_limit_and_gather_beyond(ctx):
-    """3584 is the largest side of the LDS-tiled projector: it and the gather
-    projector both match the oracle there; 3600 is refused for the LDS-tiled
-    projector and runs on the gather projector."""
+    """3840 (64 tiles of 60 samples) is the largest side of the LDS-tiled
+    projector: it and the gather projector both match the oracle there; 3856
+    is refused for the LDS-tiled projector and runs on the gather projector."""
     from ctrecon import CtrError
     rng = np.random.default_rng(1)
-    for n in (3584, 3600):
+    for n in (3840, 3856):
         pa, ss = 40, 10
         ang, g = _geom(ctx, n, n, pa, ss)
         img = rng.uniform(0.0, 1.0, (1, n, n)).astype(np.float32)
         zeros = np.zeros((1, pa, n), np.float32)
         outs = {}
         for rays in (0, 64):
-            if n > 3584 and rays == 0:
+            if n > 3840 and rays == 0:
                 with pytest.raises(CtrError):
                     ctx.set_schedule(n, 0, 4, 8)
                 continue
@@ -92,8 +92,8 @@ def test_lds_projector_limit_and_gather_beyond(ctx):
             e = relerr(got, want)
             print(f"[parity] side {n} rays {rays}: relerr vs FP32 compat {e:.3e}", flush=True)
             assert e < 1e-5
-        if n == 3584:
-            assert relerr(outs[0], outs[64]) < 1e-5   # two summation orders over 3584 samples
+        if n == 3840:
+            assert relerr(outs[0], outs[64]) < 1e-5   # two summation orders over 3840 samples
     with pytest.raises(CtrError):
         ctx.set_schedule(4096, 0, 4, 8)
 
