@@ -1405,7 +1405,10 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
     HIPTRY(c, hipSetDevice(c->device));
     const int na = c->g.pa / c->g.ss;
     const int reps = 5;
-    const int copies = 8;   // several descriptors on one slice: launches of batch size
+    // several descriptors on one slice, so every schedule is timed on a launch
+    // of batch size: about eight 1024^2 slices of work (128 copies of a 256^2
+    // slice), which fills the device whatever the side
+    const int copies = (int)std::max(8LL, std::min(256LL, 8LL * 1024 * 1024 / ((long long)n * n)));
     // a synthetic slice: disk image, ones sinogram
     const Layout l = make_layout(n, n, n);
     DevBuf pimg, pimgT, rows, prj;
