@@ -63,7 +63,7 @@ struct SliceSet {
 };
 
 struct Layout {
-    int nx, ny, px, ipitch, tpitch, rpitch;
+    int nx, ny, px, ipitch, tpitch, rpitch, roff;
 };
 
 Layout make_layout(int nx, int ny, int px)
@@ -72,7 +72,12 @@ Layout make_layout(int nx, int ny, int px)
     l.nx = nx; l.ny = ny; l.px = px;
     l.ipitch = round_up(nx + 2 * PAD, 16);
     l.tpitch = round_up(ny + 2 * PAD, 16);
-    l.rpitch = round_up(px + 4, 16);
+    // Rows hold bin b at entry roff + b with zeros around: a pixel's bin is
+    // within hypot(nx, ny)/2 + 1 of px/2, so roff zeros on either side cover
+    // every bin the backprojectors can compute, and they read out-of-range
+    // bins as 0 (the reference's border) without a clamp.
+    l.roff = std::max(1, (int)std::ceil(0.5 * std::hypot((double)nx, (double)ny) - 0.5 * px) + 2);
+    l.rpitch = round_up(px + 2 * l.roff, 16);
     return l;
 }
 
@@ -293,7 +298,7 @@ ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
     for (int z = 0; z < nz; ++z) {
         SliceDev &d = S.h[z];
         d.nx = g.nx; d.ny = g.ny; d.px = g.px;
-        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch; d.roff = l.roff;
         d.img = b.pimg ? b.pimg + (size_t)z * pimg_floats(l) : nullptr;
         d.imgT = b.pimgT ? b.pimgT + (size_t)z * pimgT_floats(l) : nullptr;
         d.rows = b.rows ? b.rows + (size_t)z * na * l.rpitch : nullptr;
@@ -696,11 +701,11 @@ extern "C" ctr_status ctr_fbp_execution(ctr_ctx *c, float *img, float *prj, int 
     launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
     POST(c, "fill rows");
     if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
-    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, g.zp, static_cast<const float2 *>(c->tw.p),
-                      c->stream);
+    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, l.roff, g.zp,
+                      static_cast<const float2 *>(c->tw.p), c->stream);
     POST(c, "FBP filter");
     launch_fbp_bp(c->w_rows.f(), img, static_cast<const float2 *>(c->cs.p), dN, g.nx, g.ny, g.pa, g.px, l.rpitch,
-                  c->stream);
+                  l.roff, c->stream);
     POST(c, "backProjectionFBP");
     return CTR_OK;
 }
@@ -1086,8 +1091,8 @@ ctr_status op_fp(ctr_ctx *c, int var, const float *img, const float *prj, float 
     TRY(op_stack(c, dN, img, prj, nullptr, true));
     TRY(run_fp(c, c->set_a, var, sub, 0, 0));
     const Layout l = make_layout(c->g.nx, c->g.ny, c->g.px);
-    launch_rows_to_full(c->w_rows.f(), out_full, dN, c->g.pa / c->g.ss, l.rpitch, c->g.pa, c->g.px, sub, c->g.ss,
-                        c->stream);
+    launch_rows_to_full(c->w_rows.f(), out_full, dN, c->g.pa / c->g.ss, l.rpitch, l.roff, c->g.pa, c->g.px, sub,
+                        c->g.ss, c->stream);
     POST(c, "rows to full");
     return CTR_OK;
 }
@@ -1101,7 +1106,7 @@ ctr_status op_bp(ctr_ctx *c, int var, const float *img, const float *rows_full, 
     TRY(op_stack(c, dN, img, raw ? rows_full : nullptr, dst, img != nullptr));
     if (!raw) {
         const Layout l = make_layout(c->g.nx, c->g.ny, c->g.px);
-        launch_full_to_rows(rows_full, c->w_rows.f(), dN, c->g.pa / c->g.ss, l.rpitch, c->g.pa, c->g.px, sub,
+        launch_full_to_rows(rows_full, c->w_rows.f(), dN, c->g.pa / c->g.ss, l.rpitch, l.roff, c->g.pa, c->g.px, sub,
                             c->g.ss, c->stream);
         POST(c, "full to rows");
     }
@@ -1169,11 +1174,11 @@ extern "C" ctr_status ctr_op_fbp_filter(ctr_ctx *c, const float *prj, float *fpr
     HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)dN * g.pa * l.rpitch));
     launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
     POST(c, "fill rows");
-    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, g.zp, static_cast<const float2 *>(c->tw.p),
-                      c->stream);
+    launch_fbp_filter(prj, c->w_rows.f(), dN, g.pa, g.px, l.rpitch, l.roff, g.zp,
+                      static_cast<const float2 *>(c->tw.p), c->stream);
     POST(c, "FBP filter");
     // all rows are "subset 0 of 1": copy the padded rows back out
-    launch_rows_to_full(c->w_rows.f(), fprj, dN, g.pa, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    launch_rows_to_full(c->w_rows.f(), fprj, dN, g.pa, l.rpitch, l.roff, g.pa, g.px, 0, 1, c->stream);
     POST(c, "rows to full");
     return CTR_OK;
 }
@@ -1186,10 +1191,10 @@ extern "C" ctr_status ctr_op_fbp_backprojection(ctr_ctx *c, const float *fprj, f
     HIPTRY(c, c->w_rows.ensure(sizeof(float) * (size_t)dN * g.pa * l.rpitch));
     launch_fill(c->w_rows.f(), (size_t)dN * g.pa * l.rpitch, 0.0f, c->stream);
     POST(c, "fill rows");
-    launch_full_to_rows(fprj, c->w_rows.f(), dN, g.pa, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    launch_full_to_rows(fprj, c->w_rows.f(), dN, g.pa, l.rpitch, l.roff, g.pa, g.px, 0, 1, c->stream);
     POST(c, "full to rows");
     launch_fbp_bp(c->w_rows.f(), img, static_cast<const float2 *>(c->cs.p), dN, g.nx, g.ny, g.pa, g.px, l.rpitch,
-                  c->stream);
+                  l.roff, c->stream);
     POST(c, "backProjectionFBP");
     return CTR_OK;
 }
@@ -1202,7 +1207,7 @@ extern "C" ctr_status ctr_op_power_iter1(ctr_ctx *c, const float *img, float *pr
     TRY(run_fp(c, c->set_a, FP_RAW, 0, 1, 1));
     // compact rows of layer s go to rows 0..na-1 of layer s (FISTA.cl:61)
     const Layout l = make_layout(g.nx, g.ny, g.px);
-    launch_rows_to_full(c->w_rows.f(), prjc, g.ss, g.pa / g.ss, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    launch_rows_to_full(c->w_rows.f(), prjc, g.ss, g.pa / g.ss, l.rpitch, l.roff, g.pa, g.px, 0, 1, c->stream);
     POST(c, "rows to full");
     return CTR_OK;
 }
@@ -1213,7 +1218,7 @@ extern "C" ctr_status ctr_op_power_iter2(ctr_ctx *c, const float *prjc, float *c
     const ctr_geometry &g = c->g;
     TRY(op_stack(c, g.ss, nullptr, nullptr, cnd, false));
     const Layout l = make_layout(g.nx, g.ny, g.px);
-    launch_full_to_rows(prjc, c->w_rows.f(), g.ss, g.pa / g.ss, l.rpitch, g.pa, g.px, 0, 1, c->stream);
+    launch_full_to_rows(prjc, c->w_rows.f(), g.ss, g.pa / g.ss, l.rpitch, l.roff, g.pa, g.px, 0, 1, c->stream);
     POST(c, "full to rows");
     TRY(run_bp(c, c->set_a, BP_POWER, 0, 1, 0.0f, nullptr, 1));
     return CTR_OK;
@@ -1314,7 +1319,7 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
             const Layout &l = lay[i];
             SliceDev &d = P.set.h[r];
             d.nx = d.ny = d.px = slices[i].n;
-            d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+            d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch; d.roff = l.roff;
             d.img = P.pimg.f() + oi;
             d.imgT = P.pimgT.f() + ot;
             d.rows = P.rows.f() + orr;
@@ -1426,7 +1431,7 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
     S.h.assign(copies, SliceDev{});
     for (SliceDev &d : S.h) {
         d.nx = d.ny = d.px = n;
-        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch;
+        d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch; d.roff = l.roff;
         d.img = pimg.f(); d.imgT = pimgT.f(); d.rows = rows.f(); d.prj = prj.f();
     }
     hipEvent_t e0, e1;

@@ -5,16 +5,17 @@
 //          ipitch floats, (ny + 2 PAD) rows.  The padding lets the projectors
 //          read without bounds tests: a direct bilinear gather clamps its
 //          coordinates into the padded frame, where any clamped sample lands
-//          on zeros (exactly CLK_ADDRESS_CLAMP's border value), and an LDS
-//          tile box (at most 97 x 94 texels) that meets the interior lies
-//          entirely inside the frame because PAD >= 97.
+//          on zeros (exactly CLK_ADDRESS_CLAMP's border value), and the
+//          96 x 92 window of an LDS tile, whose box meets the interior and so
+//          starts at most 94 texels before it, lies entirely inside the frame.
 //   pimgT  the same image transposed (pitch tpitch, nx + 2 PAD rows).  Rays
 //          steeper than 45 degrees read it so that the 64 lanes of a wave,
 //          which hold 64 neighbouring detector bins, touch consecutive
 //          addresses.
-//   rows   compact subset rows [pa/ss][rpitch]: entry 1 + X holds bin X; every
-//          other entry stays zero, so nearest/linear bin reads clamped to
-//          [-1, px] land on zeros.
+//   rows   compact subset rows [pa/ss][rpitch]: entry roff + X holds bin X;
+//          every other entry stays zero.  roff >= hypot(nx, ny)/2 - px/2 + 2,
+//          so every bin a pixel maps to lands on the row or on its zeros and
+//          the nearest-bin backprojectors read it with no clamp.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,12 +32,13 @@ struct SliceDev {
     float *img;          // padded image (pitch ipitch, ny + 2 PAD rows)
     float *imgT;         // padded transposed image (pitch tpitch, nx + 2 PAD rows)
     const float *prj;    // sinogram [pa][px], unpadded
-    float *rows;         // compact subset rows [pa/ss][rpitch]
+    float *rows;         // compact subset rows [pa/ss][rpitch], bin b at entry roff + b
     const float *src;    // unpadded [ny][nx] source of pad_in
     float *dst;          // unpadded [ny][nx] destination of unpad_out
     float *aux;          // unpadded [ny][nx] output plane of to_aux backprojections
     int nx, ny, px;
     int ipitch, tpitch, rpitch;
+    int roff;            // zeros before bin 0 (and at least as many after bin px - 1) in a row
     int layer;           // index in its stack (subset index for the power iteration)
     int blk[6];          // first logical block of this slice for each grid kind (the
                          // G_FP map holds (slice, block) pairs instead)
@@ -108,13 +110,13 @@ void launch_focus_index(const float *img, int nz, int nx, int ny, const float *s
                         float2 *partial, float *findex, hipStream_t stream);
 void launch_power3(const float *cnd, const float *l2, float *pimg, int nz, int nx, int ny, int ipitch,
                    hipStream_t stream);
-void launch_fbp_filter(const float *prj, float *frows, int nz, int pa, int px, int rpitch, int zp,
+void launch_fbp_filter(const float *prj, float *frows, int nz, int pa, int px, int rpitch, int roff, int zp,
                        const float2 *tw, hipStream_t stream);
 void launch_fbp_bp(const float *frows, float *img, const float2 *cs, int nz, int nx, int ny, int pa,
-                   int px, int rpitch, hipStream_t stream);
-void launch_rows_to_full(const float *rows, float *full, int nz, int na, int rpitch, int pa, int px,
+                   int px, int rpitch, int roff, hipStream_t stream);
+void launch_rows_to_full(const float *rows, float *full, int nz, int na, int rpitch, int roff, int pa, int px,
                          int sub, int ss, hipStream_t stream);
-void launch_full_to_rows(const float *full, float *rows, int nz, int na, int rpitch, int pa, int px,
+void launch_full_to_rows(const float *full, float *rows, int nz, int na, int rpitch, int roff, int pa, int px,
                          int sub, int ss, hipStream_t stream);
 void launch_align_forward(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,
                           const float *shifts, float center, float *prj, int nz, int nx, int ny, int pa,
