@@ -261,7 +261,7 @@ def joint_main(args):
         "ms_per_step": round(iter_s * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded 10-ellipse phantoms, analytic sinograms with per-projection shifts N(0, 2) px, "
-                "a centre offset U(-3, 3) and Poisson noise at I0=5e3 (the C5 configuration)",
+                "a centre offset U(-3, 3) and Poisson noise at I0=5e3",
         "config": {"workload": "joint FISTA-TV + detector-offset Gauss-Newton over uneven slices (one context per "
                                "side); step = one iteration over all slices", "slices_per_side": counts, "pa": pa,
                    "iterations": args.steps, "total_s_with_setup": round(total, 2)},

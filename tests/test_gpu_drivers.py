@@ -41,19 +41,19 @@ def test_osem_execution(ctx, n, pa, ss, it, nz, corr):
     assert e32 < 1e-4
 
 
-def test_osem_config_c2_full(ctx):
+def test_osem_512sq_900proj_20it(ctx):
     """512^2, 900 angles, 10 subsets, 20 iterations, 30 ellipses, Poisson I0=1e4."""
     n, pa, ss, it = 512, 900, 10, 20
     ang, g = _geom(ctx, n, pa, ss)
     img0, sino, _ = phantom_stack(1, n, pa, 30, seed=2, i0=1e4)
     x0 = np.ones_like(img0)
-    say("osem_execution C2", n, pa, ss, it)
+    say("osem_execution 512^2 x 900", n, pa, ss, it)
     x = dev(x0)
     ctx.osem_execution(x, dev(sino), it)
     got = host(x)
     c32 = O.osem_execution(g, ang, x0, sino, it)
     e32 = relerr(got, c32)
-    print(f"[parity] C2 OS-EM 512^2/900/S10/it20: relerr vs FP32 compat {e32:.3e}", flush=True)
+    print(f"[parity] OS-EM 512^2/900/S10/it20: relerr vs FP32 compat {e32:.3e}", flush=True)
     assert e32 < 1e-4
 
 
@@ -74,18 +74,18 @@ def test_aart_execution(ctx, mode):
     assert e32 < 1e-4
 
 
-def test_fbp_config_c1(ctx):
+def test_fbp_256sq_360proj(ctx):
     """256^2, 360 angles, 256 bins, Ram-Lak, ZP=512, 10 ellipses, analytic sinogram."""
     n, pa = 256, 360
     ang, g = _geom(ctx, n, pa, 1, zp=512)
     img0, sino, _ = phantom_stack(1, n, pa, 10, seed=1)
-    say("fbp_execution C1")
+    say("fbp_execution 256^2 x 360")
     x = dev(np.zeros_like(img0))
     ctx.fbp_execution(x, dev(sino))
     got = host(x)
     c32 = O.fbp_execution(g, ang, sino)
     c64 = O.fbp_execution(g, ang, sino, dtype=np.float64)
-    e32, _ = _report("fbp C1", got, c32, c64)
+    e32, _ = _report("fbp 256^2 x 360", got, c32, c64)
     assert e32 < 1e-4
 
 
