@@ -211,25 +211,28 @@ ctr_status upload_set(ctr_ctx *c, SliceSet &S)
         }
         d.blk[G_FP] = 0;
     }
-    // Forward projection: a block of a slice with D samples per ray does work
-    // proportional to D, so equal block counts are not equal work.  The block
-    // sequence is cut into 8 segments of equal work (a slice may straddle two),
-    // logical segment x runs on XCD x (xcd_remap), and the segments are padded
-    // to one length with empty entries.  Entries are (slice, block within the
-    // slice) pairs; slice -1 is empty.
+    // Forward projection: a block of a slice with D samples per ray takes time
+    // a D + b, so equal block counts are not equal work.  Fitting the
+    // LDS-tiled projector's rates at D = 256 and 1024 gives b / a ~ 93 samples
+    // (the block's tile list, box loads and epilogue).  The block sequence is
+    // cut into 8 segments of equal work (a slice may straddle two), logical
+    // segment x runs on XCD x (xcd_remap), and the segments are padded to one
+    // length with empty entries.  Entries are (slice, block within the slice)
+    // pairs; slice -1 is empty.
     {
+        constexpr double BLOCK_COST_SAMPLES = 93.0;
         std::vector<int> cnt(n);
         double total = 0.0;
         for (int s = 0; s < n; ++s) {
             const SliceDev &d = S.h[s];
             cnt[s] = fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays);
-            total += (double)cnt[s] * std::max(d.nx, d.ny);
+            total += (double)cnt[s] * (std::max(d.nx, d.ny) + BLOCK_COST_SAMPLES);
         }
         std::vector<std::vector<std::pair<int, int>>> seg(8);
         double acc = 0.0;
         int x = 0;
         for (int s = 0; s < n; ++s) {
-            const double w = std::max(S.h[s].nx, S.h[s].ny);
+            const double w = std::max(S.h[s].nx, S.h[s].ny) + BLOCK_COST_SAMPLES;
             for (int b = 0; b < cnt[s]; ++b) {
                 while (x < 7 && acc + 0.5 * w > total * (x + 1) / 8.0) ++x;
                 seg[x].push_back({s, b});
