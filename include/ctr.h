@@ -307,7 +307,10 @@ ctr_status ctr_kernel_timing(ctr_ctx *ctx, double *fp_ms, double *bp_ms, int *fp
  *   xi = (X - px/2) - shifts[a] - center.  shifts may be NULL (all zero). */
 ctr_status ctr_align_forward(ctr_ctx *ctx, const float *img, const float *shifts,
                              float center, float *prj, int dN);
-/* Exact adjoint of ctr_align_forward with respect to the image. */
+/* Adjoint of ctr_align_forward with respect to the image: the transpose of
+ * the forward's bilinear sampling weights (1 - fract, fract) of each sample's
+ * coordinates, exact except in the second image column / row, where one
+ * weight can differ by at most 2^-25.  A deterministic per-pixel gather. */
 ctr_status ctr_align_adjoint(ctr_ctx *ctx, const float *prj, const float *shifts,
                              float center, float *img, int dN);
 /* gshift[a] = d/d shifts[a] of sum(gprj * forward(img)); summed over the dN
