@@ -550,8 +550,8 @@ extern "C" ctr_status ctr_set_geometry(ctr_ctx *c, const ctr_geometry *g, const 
                     g->ny, g->px, g->pa, g->ss);
     if (g->nx > 16384 || g->ny > 16384 || g->px > 16384 || g->pa > (1 << 20))
         return fail(c, CTR_INVALID_GEOMETRY, "sizes beyond the supported maximum (16384 px, 2^20 angles)");
-    if (g->ox != g->nx || g->oy != g->ny)
-        return fail(c, CTR_INVALID_GEOMETRY, "output ROI (%d x %d) must equal the grid (%d x %d)", g->ox, g->oy,
+    if (g->ox < 1 || g->ox > g->nx || g->oy < 1 || g->oy > g->ny)
+        return fail(c, CTR_INVALID_GEOMETRY, "output ROI (%d x %d) outside the grid (%d x %d)", g->ox, g->oy,
                     g->nx, g->ny);
     if (g->pa % g->ss != 0)
         return fail(c, CTR_INVALID_GEOMETRY, "angle count %d is not a multiple of the subset count %d", g->pa,
@@ -587,11 +587,28 @@ extern "C" ctr_status ctr_set_geometry(ctr_ctx *c, const ctr_geometry *g, const 
 // ===========================================================================
 // execution drivers
 
+// An output ROI smaller than the grid is an FBP feature only.  The iterative
+// drivers' "extent reconst image" copies (GPU.cpp:199-211, 343-355,
+// 1177-1189, 1866-1878) fill the image outside the centred ox x oy block only along the
+// centred rows, and on the left only down to the last copy origin >= -ox/2;
+// the rest of the device image is never written before the projector reads
+// it.  Their OSEM2 / AART2 also update the top-left ox x oy block (an ox x oy
+// range over full-grid coordinates) while the copy back takes the centred one.
+static ctr_status require_full_grid(ctr_ctx *c)
+{
+    const ctr_geometry &g = c->g;
+    if (g.ox != g.nx || g.oy != g.ny)
+        return fail(c, CTR_INVALID_GEOMETRY,
+                    "output ROI %d x %d smaller than the grid %d x %d: only FBP supports it", g.ox, g.oy, g.nx, g.ny);
+    return CTR_OK;
+}
+
 // OSEM_execution, GPU.cpp:125-240: [sinogramCorrection] then for it, for k:
 // OSEM1(sub[k]) -> OSEM2(sub[k]) -> image update.
 extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int dN, int it, int prj_correction)
 {
     TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
     const ctr_geometry &g = c->g;
     HIPTRY(c, hipSetDevice(c->device));
     TRY(ensure_ws(c, dN, false, 0));
@@ -625,6 +642,7 @@ extern "C" ctr_status ctr_fista_osem_execution(ctr_ctx *c, float *img, float *pr
                                                int prj_correction)
 {
     TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
     const ctr_geometry &g = c->g;
     HIPTRY(c, hipSetDevice(c->device));
     TRY(ensure_ws(c, dN, false, 1));
@@ -665,6 +683,7 @@ extern "C" ctr_status ctr_fista_osem_execution(ctr_ctx *c, float *img, float *pr
 extern "C" ctr_status ctr_aart_execution(ctr_ctx *c, float *img, float *prj, int dN, int it)
 {
     TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
     const ctr_geometry &g = c->g;
     HIPTRY(c, hipSetDevice(c->device));
     TRY(ensure_ws(c, dN, false, 0));
@@ -690,7 +709,10 @@ extern "C" ctr_status ctr_aart_execution(ctr_ctx *c, float *img, float *prj, int
 }
 
 // FBP_execution, GPU.cpp:665-819: [sinogramCorrection], ramp filter of every
-// row (FBP.cl:34-160), backProjectionFBP (FBP.cl:164-188).
+// row (FBP.cl:34-160), backProjectionFBP (FBP.cl:164-188).  Every pixel of the
+// grid is written; with an output ROI the reference computes only the
+// top-left ox x oy block (GPU.cpp:704), the block FBP_thread reads back
+// (GPU.cpp:848-870), and its values are the same as here.
 extern "C" ctr_status ctr_fbp_execution(ctr_ctx *c, float *img, float *prj, int dN, int prj_correction)
 {
     TRY(check_stack_args(c, img, prj, dN));
@@ -724,6 +746,7 @@ extern "C" ctr_status ctr_hybrid_execution(ctr_ctx *c, float *img, float *prj, i
                                            int baseup_order, int cs)
 {
     TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
     if (baseup_order < 0 || baseup_order > 64) return fail(c, CTR_INVALID_ARGUMENT, "base-up order %d", baseup_order);
     TRY(ctr_fbp_execution(c, img, prj, dN, prj_correction));
     const ctr_geometry &g = c->g;
@@ -1002,6 +1025,7 @@ extern "C" ctr_status ctr_fista_execution(ctr_ctx *c, const float *L2norm, float
                                           int prj_correction)
 {
     TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
     if (!L2norm) return fail(c, CTR_INVALID_ARGUMENT, "null L2norm");
     const ctr_geometry &g = c->g;
     HIPTRY(c, hipSetDevice(c->device));

@@ -228,13 +228,17 @@ class Context:
         self._run(self.lib.ctr_set_sync_each_kernel, (self.h, int(bool(on)),), "ctr_set_sync_each_kernel")
 
     def set_geometry(self, nx, px, pa, ss=1, *, ny=None, zp=None, mode=6, correction_mode=0, wt1=1.0, wt2=0.01,
-                     cs_lambda=1e-4, angles_deg=None, sub=None):
+                     cs_lambda=1e-4, angles_deg=None, sub=None, ox=None, oy=None):
+        """ox, oy: the output region (g_ox, g_oy), the grid size by default;
+        a smaller one is accepted by fbp_execution only."""
         ny = nx if ny is None else ny
+        ox = nx if ox is None else ox
+        oy = ny if oy is None else oy
         if zp is None:
             zp = 1
             while zp < 2 * px:
                 zp *= 2
-        g = Geometry(nx, ny, nx, ny, px, pa, ss, zp, mode, correction_mode, wt1, wt2, cs_lambda)
+        g = Geometry(nx, ny, ox, oy, px, pa, ss, zp, mode, correction_mode, wt1, wt2, cs_lambda)
         ang = np.ascontiguousarray(angles_deg, dtype=np.float32)
         if ang.shape != (pa,):
             raise ValueError("angles_deg must hold pa values")

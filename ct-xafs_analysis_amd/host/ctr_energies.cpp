@@ -284,9 +284,10 @@ int run_energies(const EnergyParams &P, std::ostream &log)
             const std::string dir = P.output_dir + "/" + subs[k];
             if (::mkdir(dir.c_str(), 0755) != 0 && errno != EEXIST) return "failed to create directory " + dir;
             std::string we;
+            std::vector<float> roi;
             for (const auto &rg : ranges)
                 for (int N = rg.first; N <= rg.second; ++N)
-                    if (!write_raw(dir, P.output_base + tag4(N) + ".raw", src + img_n * (N - first),
+                    if (!write_raw(dir, P.output_base + tag4(N) + ".raw", output_layer(p, src + img_n * (N - first), roi),
                                    (size_t)p.ox * p.oy, &we))
                         return we;
             const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -82,6 +82,10 @@ bool read_his_rows(const std::string &path, int start_frame, int end_frame, int 
 
 // fileIO.cpp:55-68: the initial image (file for fst == 1, constant ini for fst == 2).
 bool first_image(const Params &p, float *img, size_t count, std::string *err);
+// The output layer of an nx x ny image: the image itself, or with an output
+// ROI (FBP only) its top-left ox x oy block, the region FBP_thread reads back
+// from origin (0, 0) (GPU.cpp:848-870), copied into `tmp`.
+const float *output_layer(const Params &p, const float *img, std::vector<float> &tmp);
 
 // Effective subset count of each method (GPU.cpp:36-93: MART uses one
 // subset per projection, ASIRT / MSIRT / ML-EM one subset).

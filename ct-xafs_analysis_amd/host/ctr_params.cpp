@@ -3,6 +3,7 @@
 // main.cpp of the reference).
 #include "ctr_host.hpp"
 
+#include <algorithm>
 #include <cerrno>
 #include <cstring>
 #include <fstream>
@@ -219,6 +220,15 @@ bool read_raw(const std::string &dir, const std::string &name, float *data, size
     return true;
 }
 
+const float *output_layer(const Params &p, const float *img, std::vector<float> &tmp)
+{
+    if (p.ox == p.nx && p.oy == p.ny) return img;
+    tmp.resize((size_t)p.ox * p.oy);
+    for (int y = 0; y < p.oy; ++y)
+        std::copy(img + (size_t)y * p.nx, img + (size_t)y * p.nx + p.ox, tmp.data() + (size_t)y * p.ox);
+    return tmp.data();
+}
+
 bool write_raw(const std::string &dir, const std::string &name, const float *data, size_t count,
                std::string *err)
 {
@@ -331,8 +341,14 @@ bool validate(const Params &p, std::string *err)
         set_err(err, "reconstruction method " + std::to_string(p.mode) + " is not 1-8");
         return false;
     }
-    if (p.ox != p.nx || p.oy != p.ny) {
-        set_err(err, "output size must equal the reconstruction size (ROI extension is not provided)");
+    // an output ROI is read back from FBP only (the top-left ox x oy block,
+    // GPU.cpp:848-870); the iterative methods' ROI extension is not provided
+    if (p.ox < 1 || p.ox > p.nx || p.oy < 1 || p.oy > p.ny) {
+        set_err(err, "output size must be within the reconstruction size");
+        return false;
+    }
+    if ((p.ox != p.nx || p.oy != p.ny) && p.mode != 7) {
+        set_err(err, "output size must equal the reconstruction size except for FBP (ROI extension is not provided)");
         return false;
     }
     if (p.nx < 2 || p.ny < 2 || p.px < 2 || p.pa < 1 || p.num < 1 || p.it < 0 || p.numParallel < 1) {

@@ -177,8 +177,11 @@ int run(const Params &p, std::ostream &log)
     // output_thread (GPU.cpp:99-123): writes layers [start + offset, end]
     auto write_batch = [&](HostBatch b, int offset, int out_end) -> std::string {
         std::string e;
+        std::vector<float> roi;
         for (int i = b.c.start + offset; i <= out_end; ++i)
-            if (!write_raw(p.d2, layer_name(p.f2, i), &b.imgs[img_n * (i - b.c.start)], img_n, &e)) return e;
+            if (!write_raw(p.d2, layer_name(p.f2, i), output_layer(p, &b.imgs[img_n * (i - b.c.start)], roi),
+                           (size_t)p.ox * p.oy, &e))
+                return e;
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::lock_guard<std::mutex> lk(g_log_mu);
         for (int i = b.c.start + offset; i <= out_end; ++i)

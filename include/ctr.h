@@ -44,7 +44,8 @@ typedef enum ctr_status {
  * program with -D (GPU.cpp:538-547, 948-954, 1336-1343, 2051-2059). */
 typedef struct ctr_geometry {
     int nx, ny;           /* g_nx, g_ny: reconstruction grid */
-    int ox, oy;           /* g_ox, g_oy: output ROI; must equal nx, ny */
+    int ox, oy;           /* g_ox, g_oy: output ROI, 1..nx x 1..ny; below the grid
+                             size for ctr_fbp_execution only */
     int px;               /* g_px: detector bins */
     int pa;               /* g_pa: projection angles */
     int ss;               /* g_ss: ordered subsets; pa % ss == 0 */
@@ -110,7 +111,12 @@ ctr_status ctr_fista_osem_execution(ctr_ctx *ctx, float *reconst_img, float *prj
 ctr_status ctr_aart_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
                               int dN, int it);
 
-/* FBP_execution, GPU.cpp:665-819 (log2(zp) stages, GPU.cpp:855). */
+/* FBP_execution, GPU.cpp:665-819 (log2(zp) stages, GPU.cpp:855).  Writes the
+ * whole nx x ny grid of each layer.  With an output ROI (ox < nx or oy < ny)
+ * the reference's FBP_thread returns the top-left ox x oy block of it
+ * (backProjectionFBP over an ox x oy range in full-grid coordinates,
+ * GPU.cpp:704, read back from origin (0, 0), GPU.cpp:848-870).  The iterative
+ * drivers reject an ROI (CTR_INVALID_GEOMETRY). */
 ctr_status ctr_fbp_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
                              int dN, int prj_correction);
 

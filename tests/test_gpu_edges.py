@@ -118,6 +118,24 @@ def test_zero_and_nan_sinograms(ctx):
         assert relerr(host(xa), O.aart_execution(g, ang, np.ones((1, n, n), np.float32), yy, 1, 1.0)) < 1e-4, name
 
 
+def test_fbp_with_output_region(ctx):
+    """An output region below the grid size changes nothing in the FBP
+    image: the whole grid is written and its top-left ox x oy block is the
+    reference's output (GPU.cpp:704, 848-870)."""
+    n, pa = 64, 90
+    _, sinos, ang = phantom_stack(2, n, pa, 5, seed=3)
+    ctx.set_geometry(n, n, pa, 1, zp=128, angles_deg=ang)
+    full = dev(np.zeros((2, n, n)))
+    ctx.fbp_execution(full, dev(sinos))
+    ctx.set_geometry(n, n, pa, 1, zp=128, angles_deg=ang, ox=40, oy=24)
+    roi = dev(np.zeros((2, n, n)))
+    say("FBP with a 40 x 24 output region")
+    ctx.fbp_execution(roi, dev(sinos))
+    assert np.array_equal(host(roi), host(full))
+    want = O.fbp_execution(O.geom(n, n, pa, zp=128), ang, sinos)
+    assert relerr(host(roi)[:, :24, :40], want[:, :24, :40]) < 1e-5
+
+
 def test_error_statuses_leave_the_context_usable(ctx):
     import torch
     from ctrecon import CtrError
@@ -135,6 +153,9 @@ def test_error_statuses_leave_the_context_usable(ctx):
         lambda: ctx.set_geometry(0, n, pa, ss, angles_deg=ang),           # empty grid
         lambda: ctx.set_schedule(n, 12, 4, 8),                             # no such schedule
         lambda: ctx.osem_batched([], [], 1),                               # empty batch
+        lambda: ctx.set_geometry(n, n, pa, ss, angles_deg=ang, ox=n + 2),  # output region beyond the grid
+        lambda: (ctx.set_geometry(n, n, pa, ss, angles_deg=ang, ox=n - 8),  # output region: FBP only
+                 ctx.osem_execution(x, y, 1)),
     ]
     for i, f in enumerate(bad):
         say("bad call", i)

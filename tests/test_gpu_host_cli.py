@@ -83,6 +83,23 @@ def test_fbp(tmp_path):
     assert relerr(outputs(tmp_path, nz), want) < 1e-5
 
 
+def test_fbp_output_region(tmp_path):
+    """An output region below the grid size: backProjectionFBP runs over an
+    ox x oy range in full-grid coordinates and FBP_thread reads that top-left
+    block back (GPU.cpp:704, 848-870), so each file holds oy rows of ox
+    values of the full-grid image."""
+    nz, ox, oy = 2, 30, 20
+    inp, sinos, ang = dataset(tmp_path, nz, [("#再構成法", "7"), ("#ゼロパディングサイズ(FBP)", "128"),
+                                             ("#出力画像サイズ X", str(ox)), ("#出力画像サイズ Y", str(oy))])
+    run_cli(inp, "FBP output region")
+    want = O.fbp_execution(O.geom(N, N, PA, zp=128), ang, sinos)[:, :oy, :ox]
+    got = [np.fromfile(tmp_path / "out" / f"r{z + 1:04d}.img", np.float32) for z in range(nz)]
+    assert all(g.size == ox * oy for g in got)
+    e = relerr(np.stack(got).reshape(nz, oy, ox), want)
+    print(f"[parity] CLI FBP output region {ox} x {oy}: relerr {e:.3e}", flush=True)
+    assert e < 1e-5 and np.abs(want).max() > 0
+
+
 @pytest.mark.parametrize("mode,alpha", [(1, 1.0), (3, 0.01)])
 def test_aart_asirt(tmp_path, mode, alpha):
     nz = 3

@@ -84,6 +84,22 @@ def test_fbp_over_parameter_names(tmp_path):
         assert relerr(got, want) < 1e-5
 
 
+def test_fbp_output_region_over_energies(tmp_path):
+    """FBP with an output region below the grid: each file is the top-left
+    ox x oy block of the layer (FBP_thread, GPU.cpp:848-870)."""
+    subs, ox, oy = ["001", "002"], 40, 28
+    inp, stacks, ang = energy_dataset(tmp_path, subs, [("#Energy number range", "1-2"), ("#再構成法", "7"),
+                                                        ("#ゼロパディングサイズ(FBP)", "128"),
+                                                        ("#出力画像サイズ X", str(ox)), ("#出力画像サイズ Y", str(oy))])
+    run(inp, "FBP output region")
+    g = O.geom(N, N, PA, zp=128)
+    for sub in subs:
+        want = O.fbp_execution(g, ang, stacks[sub][2:7])[:, :oy, :ox]
+        got = np.stack([np.fromfile(tmp_path / "out" / sub / f"rec_{n:04d}.raw", np.float32) for n in range(2, 7)])
+        assert got.shape == (5, ox * oy)
+        assert relerr(got.reshape(5, oy, ox), want) < 1e-5
+
+
 def test_fista_osem_batches_and_skipped_layers(tmp_path):
     """FISTA-OS-EM in batches of numParallel = 3 layers without overlap:
     [2..4], [5..6]; OSEM_thread writes a batch's layers minus CSoverlap at

@@ -148,12 +148,24 @@ def _tiny_dataset(tmp_path, extra):
     ([("#再構成法", "8"), ("#ゼロパディングサイズ(FBP)", "24")], "power of two"),
     ([("#サブセット(OSEM)", "5")], "not a multiple"),
     ([("#出力画像サイズ X", "8")], "output size"),
+    ([("#出力画像サイズ Y", "20")], "within the reconstruction size"),
+    ([("#再構成法", "1"), ("#出力画像サイズ Y", "8")], "except for FBP"),
     ([("#再構成法", "7"), ("#ゼロパディングサイズ(FBP)", "24")], "power of two"),
 ])
 def test_rejected_parameter_sets(tmp_path, extra, msg):
     inp = _tiny_dataset(tmp_path, extra)
     r = subprocess.run([CLI, str(inp)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and msg in r.stdout + r.stderr
+
+
+def test_fbp_output_region_passes_validation(tmp_path):
+    """FBP takes an output region below the grid size (its top-left block is
+    read back, GPU.cpp:848-870); the run then stops only for want of a GPU
+    here, or runs to the end where one is present."""
+    inp = _tiny_dataset(tmp_path, [("#再構成法", "7"), ("#ゼロパディングサイズ(FBP)", "32"),
+                                   ("#出力画像サイズ X", "8"), ("#出力画像サイズ Y", "6")])
+    r = subprocess.run([CLI, str(inp)], capture_output=True, text=True, timeout=120)
+    assert "output size" not in r.stdout + r.stderr
 
 
 def test_short_angle_log(tmp_path):
