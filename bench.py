@@ -51,9 +51,9 @@ PEAK_HBM_GBS = 8000.0
 # Bytes moved past L2 per forward-projection launch, per slice of 1024^2 with
 # 120 angles (one subset of 1200): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
 # the 128-slice k_fp_lds<0> launch, scaled by the same counters on a plain
-# 256 MiB copy (FETCH x 1.99970, WRITE x 1.0), divided by 128
+# 256 MiB copy (FETCH x 1.99969, WRITE x 1.0), divided by 128
 # (profiles/k_fp_lds_k_bp_osem_1024sq_1200proj_128slices.md).
-FP_TRAFFIC_BYTES_PER_SLICE = 44104512.0
+FP_TRAFFIC_BYTES_PER_SLICE = 44102141.0
 FP_TRAFFIC_SHAPE = (1024, 1200, 10)
 # Bytes moved past L2 per k_reslice launch for 1600 projections of 2048 x 2048
 # (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled the same way; the 9 rows a
