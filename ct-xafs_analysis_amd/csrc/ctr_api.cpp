@@ -525,20 +525,36 @@ extern "C" ctr_status ctr_set_sync_each_kernel(ctr_ctx *c, int enable)
     return CTR_OK;
 }
 
+// The reference's order (GPU.cpp:564-582): for ss = 2^m, position i holds
+// the m-bit reversal of i, so consecutive subsets are as far apart in angle as
+// the earlier ones allow (16 -> 0, 8, 4, 12, 2, ...); any other count runs
+// 0 .. ss - 1 in order.
 extern "C" ctr_status ctr_subset_order(int ss, int *sub)
 {
     if (ss < 1 || !sub) return CTR_INVALID_ARGUMENT;
-    // GPU.cpp:564-582
-    int bits = 0;
-    for (int i = 0; i < 32; ++i) bits += (ss >> i) & 1;
-    if (bits == 1) {
-        int m1 = 0;
-        sub[m1++] = 0;
-        for (int i = ss, m2 = 1; i > 1; i /= 2, m2 *= 2)
-            for (int j = 0; j < m2; ++j) sub[m1++] = sub[j] + i / 2;
-    } else {
-        for (int i = 0; i < ss; ++i) sub[i] = i;
+    const bool pow2 = (ss & (ss - 1)) == 0;
+    int m = 0;
+    while ((1 << m) < ss) ++m;
+    for (int i = 0; i < ss; ++i) {
+        int r = i;
+        if (pow2) {
+            r = 0;
+            for (int k = 0; k < m; ++k) r |= ((i >> k) & 1) << (m - 1 - k);
+        }
+        sub[i] = r;
     }
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_query_layout(int nx, int ny, int px, ctr_layout *out)
+{
+    if (!out || nx < 1 || ny < 1 || px < 1 || nx > 16384 || ny > 16384 || px > 16384) return CTR_INVALID_ARGUMENT;
+    const Layout l = make_layout(nx, ny, px);
+    out->pad = PAD;
+    out->ipitch = l.ipitch;
+    out->tpitch = l.tpitch;
+    out->rpitch = l.rpitch;
+    out->roff = l.roff;
     return CTR_OK;
 }
 

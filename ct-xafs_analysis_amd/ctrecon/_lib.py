@@ -38,6 +38,11 @@ class Slice(C.Structure):
     _fields_ = [("n", C.c_int), ("img", C.c_void_p), ("prj", C.c_void_p)]
 
 
+class Layout(C.Structure):
+    """ctr_layout (include/ctr.h)."""
+    _fields_ = [("pad", C.c_int), ("ipitch", C.c_int), ("tpitch", C.c_int), ("rpitch", C.c_int), ("roff", C.c_int)]
+
+
 class ResliceParams(C.Structure):
     """ctr_reslice_params (include/ctr.h)."""
     _fields_ = [("baseup", C.c_float), ("z_correction", C.c_int), ("start_x", C.c_int), ("end_x", C.c_int),
@@ -60,6 +65,7 @@ _SIGS = {
     "ctr_set_sync_each_kernel": [_P, _I],
     "ctr_set_geometry": [_P, C.POINTER(Geometry), _P, _P],
     "ctr_subset_order": [_I, _P],
+    "ctr_query_layout": [_I, _I, _I, C.POINTER(Layout)],
     "ctr_osem_execution": [_P, _P, _P, _I, _I, _I],
     "ctr_fista_osem_execution": [_P, _P, _P, _I, _I, _I],
     "ctr_aart_execution": [_P, _P, _P, _I, _I],
@@ -131,6 +137,15 @@ def subset_order(ss: int) -> np.ndarray:
     return sub
 
 
+def query_layout(nx: int, ny: int, px: int) -> Layout:
+    """The device layout of one slice (host-only; works without a GPU)."""
+    out = Layout()
+    st = load().ctr_query_layout(nx, ny, px, C.byref(out))
+    if st != CTR_OK:
+        raise CtrError(st, f"ctr_query_layout({nx}, {ny}, {px})")
+    return out
+
+
 def _ptr(t):
     if t is None:
         return None
@@ -150,6 +165,36 @@ def _ptr_u16(t):
         raise ValueError("expected a contiguous 16-bit GPU tensor")
     return C.c_void_p(t.data_ptr())
 
+
+
+def _shape_of(t):
+    return tuple(int(v) for v in t.shape)
+
+
+def _need(t, shape, what):
+    """Raises ValueError unless tensor t has exactly `shape` (checked before
+    any pointer reaches a kernel: a wrong extent would make it read or write
+    past the buffer)."""
+    if t is None:
+        raise ValueError(f"{what}: missing tensor")
+    if _shape_of(t) != tuple(shape):
+        raise ValueError(f"{what}: shape {_shape_of(t)} does not match the expected {tuple(shape)}")
+
+
+def _need_numel(t, n, what, at_least=False):
+    if t is None:
+        raise ValueError(f"{what}: missing tensor")
+    m = int(t.numel())
+    if (m < n) if at_least else (m != n):
+        raise ValueError(f"{what}: {m} elements, expected {'at least ' if at_least else ''}{n}")
+
+
+def _stack(t, what):
+    """The layer count of an image or sinogram stack [dN][.][.]."""
+    if t is None or t.dim() != 3 or t.shape[0] < 1:
+        raise ValueError(f"{what}: expected a 3-D stack [dN][rows][columns], got "
+                         f"{None if t is None else _shape_of(t)}")
+    return int(t.shape[0])
 
 
 class Context:
@@ -200,6 +245,19 @@ class Context:
         if self._ext is not None:
             cur.wait_stream(self._ext)
         self._chk(st, what)
+
+    def _g(self):
+        if self.geom is None:
+            raise ValueError("set_geometry() first")
+        return self.geom
+
+    def _img(self, t, dN, what):
+        g = self._g()
+        _need(t, (dN, g.ny, g.nx), what)
+
+    def _sino(self, t, dN, what):
+        g = self._g()
+        _need(t, (dN, g.pa, g.px), what)
 
     def _chk(self, st, what):
         if st != CTR_OK:
@@ -252,23 +310,36 @@ class Context:
         return g
 
     # ---- drivers ----
+    def _driver_args(self, img, prj, what):
+        dN = _stack(img, what + " image")
+        self._img(img, dN, what + " image")
+        self._sino(prj, dN, what + " sinogram")
+        return dN
+
     def osem_execution(self, img, prj, it, prj_correction=True):
-        self._run(self.lib.ctr_osem_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction),),
+        dN = self._driver_args(img, prj, "osem_execution")
+        self._run(self.lib.ctr_osem_execution, (self.h, _ptr(img), _ptr(prj), dN, it, int(prj_correction),),
                   "ctr_osem_execution")
 
     def fista_osem_execution(self, img, prj, it, prj_correction=True):
-        self._run(self.lib.ctr_fista_osem_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it,
+        dN = self._driver_args(img, prj, "fista_osem_execution")
+        self._run(self.lib.ctr_fista_osem_execution, (self.h, _ptr(img), _ptr(prj), dN, it,
                                                     int(prj_correction),), "ctr_fista_osem_execution")
 
     def aart_execution(self, img, prj, it):
-        self._run(self.lib.ctr_aart_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it,), "ctr_aart_execution")
+        dN = self._driver_args(img, prj, "aart_execution")
+        self._run(self.lib.ctr_aart_execution, (self.h, _ptr(img), _ptr(prj), dN, it,), "ctr_aart_execution")
 
     def hybrid_execution(self, img, prj, it, prj_correction=True, baseup_order=4, cs=False):
-        self._run(self.lib.ctr_hybrid_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], it, int(prj_correction),
+        dN = self._driver_args(img, prj, "hybrid_execution")
+        self._run(self.lib.ctr_hybrid_execution, (self.h, _ptr(img), _ptr(prj), dN, it, int(prj_correction),
                                                   baseup_order, int(bool(cs))), "ctr_hybrid_execution")
 
     def fbp_execution(self, img, prj, prj_correction=True):
-        self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), img.shape[0], int(prj_correction),),
+        """img: the whole grid [dN][ny][nx] (an output region is the caller's
+        top-left ox x oy block of it)."""
+        dN = self._driver_args(img, prj, "fbp_execution")
+        self._run(self.lib.ctr_fbp_execution, (self.h, _ptr(img), _ptr(prj), dN, int(prj_correction),),
                   "ctr_fbp_execution")
 
     # ---- sinogram formation (Reslice/) ----
@@ -277,45 +348,63 @@ class Context:
         t = self._torch
         out = t.empty((H, W), dtype=t.float32, device=raw.device)
         stride = W * H if frame_stride is None else frame_stride
+        _need_numel(raw, (n - 1) * stride + W * H, "merge_frames raw", at_least=True)
         self._run(self.lib.ctr_merge_frames, (self.h, _ptr_u16(raw), n, W, H, stride, _ptr(out)), "ctr_merge_frames")
         return out
 
     def mt_conversion(self, dark, I0, It, n, frame_stride=None):
         """mt_conversion: mu-t [n, H, W] of n 16-bit frames against dark / I0 [H, W]."""
         t = self._torch
+        if dark.dim() != 2:
+            raise ValueError(f"mt_conversion dark: expected [H, W], got {_shape_of(dark)}")
         H, W = dark.shape
-        out = t.empty((n, H, W), dtype=t.float32, device=dark.device)
+        _need(I0, (H, W), "mt_conversion I0")
         stride = W * H if frame_stride is None else frame_stride
+        _need_numel(It, (n - 1) * stride + W * H, "mt_conversion It", at_least=True)
+        out = t.empty((n, H, W), dtype=t.float32, device=dark.device)
         self._run(self.lib.ctr_mt_conversion, (self.h, _ptr(dark), _ptr(I0), _ptr_u16(It), n, W, H, stride, _ptr(out)),
                   "ctr_mt_conversion")
         return out
 
     def xprojection(self, mt, start_x, end_x):
         t = self._torch
-        A, H, W = mt.shape
+        A, H, W = _shape_of(mt) if mt.dim() == 3 else (None, None, None)
+        if A is None:
+            raise ValueError(f"xprojection: expected mt [A, H, W], got {_shape_of(mt)}")
         out = t.empty((A, H), dtype=t.float32, device=mt.device)
         self._run(self.lib.ctr_xprojection, (self.h, _ptr(mt), A, H, W, start_x, end_x, _ptr(out)), "ctr_xprojection")
         return out
 
     def zcorrection(self, xproj, start_z, end_z, yshift=None):
         t = self._torch
+        if xproj.dim() != 2:
+            raise ValueError(f"zcorrection: expected xproj [A, H], got {_shape_of(xproj)}")
         A, H = xproj.shape
         ys = t.zeros((A,), dtype=t.float32, device=xproj.device) if yshift is None else yshift
+        _need_numel(ys, A, "zcorrection yshift")
         self._run(self.lib.ctr_zcorrection, (self.h, _ptr(xproj), A, H, start_z, end_z, _ptr(ys)), "ctr_zcorrection")
         return ys
 
     def reslice(self, mt, xshift=None, yshift=None, baseup=0.0, correction=0, y0=0, n_layers=None, out=None):
         """Sinograms [n_layers, A, W] (camera rows y0 ..) from projections [A, H, W]."""
         t = self._torch
+        if mt.dim() != 3:
+            raise ValueError(f"reslice: expected mt [A, H, W], got {_shape_of(mt)}")
         A, H, W = mt.shape
         n_layers = H - y0 if n_layers is None else n_layers
+        for sh, nm in ((xshift, "xshift"), (yshift, "yshift")):
+            if sh is not None:
+                _need_numel(sh, A, "reslice " + nm)
         if out is None:
             out = t.empty((n_layers, A, W), dtype=t.float32, device=mt.device)
+        _need(out, (n_layers, A, W), "reslice out")
         self._run(self.lib.ctr_reslice, (self.h, _ptr(mt), A, H, W, _ptr(xshift), _ptr(yshift), float(baseup),
                                          int(correction), y0, n_layers, _ptr(out)), "ctr_reslice")
         return out
 
     def denoise_sinogram(self, src, threshold):
+        if src.dim() != 3:
+            raise ValueError(f"denoise_sinogram: expected [nz, A, W], got {_shape_of(src)}")
         out = self._torch.empty_like(src)
         nz, A, W = src.shape
         self._run(self.lib.ctr_denoise_sinogram, (self.h, _ptr(src), _ptr(out), nz, A, W, float(threshold)),
@@ -327,6 +416,8 @@ class Context:
         """reslice_thread: [Z-correction] -> reslice -> [spike removal].
         Returns (sinograms [n_layers, A, W], yshift [A])."""
         t = self._torch
+        if mt.dim() != 3:
+            raise ValueError(f"reslice_execution: expected mt [A, H, W], got {_shape_of(mt)}")
         A, H, W = mt.shape
         n_layers = H - y0 if n_layers is None else n_layers
         p = ResliceParams(baseup, int(bool(z_correction)), start_x, end_x, start_z, end_z, int(bool(denoise)),
@@ -334,6 +425,7 @@ class Context:
         ys = t.empty((A,), dtype=t.float32, device=mt.device)
         if out is None:
             out = t.empty((n_layers, A, W), dtype=t.float32, device=mt.device)
+        _need(out, (n_layers, A, W), "reslice_execution out")
         self._run(self.lib.ctr_reslice_execution, (self.h, _ptr(mt), A, H, W, C.byref(p), _ptr(ys), _ptr(out)),
                   "ctr_reslice_execution")
         return out, ys
@@ -345,7 +437,8 @@ class Context:
         first and last angle of the geometry.  Returns (recon [n_shift, ny, nx],
         findex [n_shift], best shift)."""
         t = self._torch
-        g = self.geom
+        g = self._g()
+        _need_numel(prj, g.pa * g.px, "rotation_center_search prj")
         if ang_range is None:
             ang_range = (float(self.angles[0]), float(self.angles[-1]))
         recon = t.empty((n_shift, g.ny, g.nx), dtype=t.float32, device=prj.device)
@@ -358,24 +451,84 @@ class Context:
         return recon, findex, np.float32(shift_start) + np.float32(shift_step) * np.float32(best)
 
     def power_iteration(self, L, trials=30):
+        _need_numel(L, self._g().ss, "power_iteration L2norm", at_least=True)
         self._run(self.lib.ctr_power_iteration, (self.h, _ptr(L), trials,), "ctr_power_iteration")
 
     def fista_execution(self, L, img, prj, it, prj_correction=True):
-        self._run(self.lib.ctr_fista_execution, (self.h, _ptr(L), _ptr(img), _ptr(prj), img.shape[0], it,
+        dN = self._driver_args(img, prj, "fista_execution")
+        _need_numel(L, self._g().ss, "fista_execution L2norm", at_least=True)
+        self._run(self.lib.ctr_fista_execution, (self.h, _ptr(L), _ptr(img), _ptr(prj), dN, it,
                                                int(prj_correction),), "ctr_fista_execution")
 
     # ---- single operators ----
+    def _op_shapes(self, name, args):
+        """(tensor, expected shape or element count, label) for every tensor
+        argument of ctr_op_<name> (include/ctr.h), from the geometry and the
+        layer count argument."""
+        g = self._g()
+        im = lambda n: (n, g.ny, g.nx)            # noqa: E731
+        sn = lambda n: (n, g.pa, g.px)            # noqa: E731
+        L = ("L", g.ss)
+        table = {
+            "osem1": (3, [im, sn, sn]), "aart1": (3, [im, sn, sn]),
+            "osem2": (3, [im, sn, im]), "aart2": (3, [im, sn, im]),
+            "backprojection": (2, [sn, im]),
+            "sinogram_correction": (2, [sn, sn]),
+            "fbp_filter": (2, [sn, sn]), "fbp_backprojection": (2, [sn, im]),
+            "fista_backprojection": (4, [im, sn, L, im]),
+            "ista3d": (2, [im, im, None, None, L]),
+            "fista3d": (6, [im, im, im, im, im, im, None, None, L]),
+        }
+        if name in ("power_iter1", "power_iter2"):
+            specs = [im, sn] if name == "power_iter1" else [sn, im]
+            return [(args[i], specs[i](g.ss), f"op {name} argument {i}") for i in range(2)]
+        if name == "rot_center_shift":
+            return [(args[0], g.pa * g.px, "op rot_center_shift prj"),
+                    (args[1], int(args[2]) * g.pa * g.px, "op rot_center_shift out")]
+        if name == "focus_index":
+            n = int(args[1])
+            return [(args[0], n * g.nx * g.ny, "op focus_index img"), (args[6], n, "op focus_index findex")]
+        if name not in table:
+            raise ValueError(f"unknown operator {name}")
+        dn_at, specs = table[name]
+        dN = int(args[dn_at])
+        if dN < 1:
+            raise ValueError(f"op {name}: dN must be >= 1")
+        out = []
+        for i, sp in enumerate(specs):
+            if sp is None:
+                continue
+            if isinstance(sp, tuple):             # L2norm: at least ss entries
+                out.append((args[i], ("min", sp[1]), f"op {name} L2norm"))
+            elif args[i] is not None or name != "fista3d" or i < 3:
+                out.append((args[i], sp(dN), f"op {name} argument {i}"))
+        return out
+
     def op(self, name, *args):
+        for t, want, what in self._op_shapes(name, args):
+            if isinstance(want, tuple) and len(want) == 2 and want[0] == "min":
+                _need_numel(t, want[1], what, at_least=True)
+            elif isinstance(want, tuple):
+                _need(t, want, what)
+            else:
+                _need_numel(t, want, what)
         f = getattr(self.lib, "ctr_op_" + name)
         conv = [(_ptr(a) if (a is None or isinstance(a, self._torch.Tensor)) else a) for a in args]
         self._run(f, (self.h, *conv), "ctr_op_" + name)
 
     # ---- batched ----
     def osem_batched(self, imgs, prjs, it, use_graph=True):
+        g = self._g()
+        if len(imgs) != len(prjs) or not imgs:
+            raise ValueError("osem_batched: need as many images as sinograms, at least one")
         n = len(imgs)
         arr = (Slice * n)()
         for i, (im, pr) in enumerate(zip(imgs, prjs)):
-            arr[i] = Slice(int(im.shape[-1]), _ptr(im).value, _ptr(pr).value)
+            m = int(im.shape[-1])
+            if im.dim() not in (2, 3) or _shape_of(im)[-2:] != (m, m) or int(im.numel()) != m * m:
+                raise ValueError(f"osem_batched slice {i}: image shape {_shape_of(im)} is not one square slice")
+            _need_numel(pr, g.pa * m, f"osem_batched slice {i} sinogram ([pa][{m}])")
+            arr[i] = Slice(m, _ptr(im).value, _ptr(pr).value)
         self._keep = (imgs, prjs, arr)
         self._run(self.lib.ctr_osem_batched, (self.h, arr, n, it, int(use_graph),), "ctr_osem_batched")
 
@@ -398,14 +551,26 @@ class Context:
         return a.value, b.value, na.value, nb.value
 
     # ---- differentiable model ----
+    def _align_args(self, img, prj, shifts, what):
+        dN = _stack(img, what + " image")
+        self._img(img, dN, what + " image")
+        self._sino(prj, dN, what + " projections")
+        if shifts is not None:
+            _need_numel(shifts, self._g().pa, what + " shifts")
+        return dN
+
     def align_forward(self, img, shifts, center, prj):
-        self._run(self.lib.ctr_align_forward, (self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj),
-                                             img.shape[0],), "ctr_align_forward")
+        dN = self._align_args(img, prj, shifts, "align_forward")
+        self._run(self.lib.ctr_align_forward, (self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj), dN,),
+                  "ctr_align_forward")
 
     def align_adjoint(self, prj, shifts, center, img):
-        self._run(self.lib.ctr_align_adjoint, (self.h, _ptr(prj), _ptr(shifts), float(center), _ptr(img),
-                                             img.shape[0],), "ctr_align_adjoint")
+        dN = self._align_args(img, prj, shifts, "align_adjoint")
+        self._run(self.lib.ctr_align_adjoint, (self.h, _ptr(prj), _ptr(shifts), float(center), _ptr(img), dN,),
+                  "ctr_align_adjoint")
 
     def align_grad_shift(self, img, gprj, shifts, center, gshift):
+        dN = self._align_args(img, gprj, shifts, "align_grad_shift")
+        _need_numel(gshift, self._g().pa, "align_grad_shift gshift")
         self._run(self.lib.ctr_align_grad_shift, (self.h, _ptr(img), _ptr(gprj), _ptr(shifts), float(center),
-                                                _ptr(gshift), img.shape[0],), "ctr_align_grad_shift")
+                                                _ptr(gshift), dN,), "ctr_align_grad_shift")

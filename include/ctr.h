@@ -97,6 +97,17 @@ ctr_status ctr_set_geometry(ctr_ctx *ctx, const ctr_geometry *geom,
 /* Writes the reference subset order for `ss` subsets (GPU.cpp:564-582). */
 ctr_status ctr_subset_order(int ss, int *sub);
 
+/* The device layout of one slice of nx x ny pixels and px bins: the zero frame
+ * (pad texels on every side) and row pitches of the padded image and of its
+ * transposed copy, and the compact subset rows' pitch with roff zeros before
+ * bin 0 (and at least as many after bin px - 1).  Host-only: no device call.
+ * No reference counterpart (OpenCL images hide their layout); for callers
+ * that size workspace and for layout checks. */
+typedef struct ctr_layout {
+    int pad, ipitch, tpitch, rpitch, roff;
+} ctr_layout;
+ctr_status ctr_query_layout(int nx, int ny, int px, ctr_layout *out);
+
 /* ---- execution drivers (drop-ins for GPU.cpp) --------------------------- */
 
 /* OSEM_execution, GPU.cpp:125-240 (declared CT_reconstruction.hpp:77). */

@@ -20,17 +20,23 @@
 
 /* GPU.cpp:564-582: if ss is a power of two the subsets are visited in
  * bit-reversed order (ss=16 -> 0,8,4,12,2,10,...), otherwise 0..ss-1. */
+/* Subset order of GPU.cpp:564-582.  For ss = 2^m the list doubles m times:
+ * with spacing `half` between the subsets placed so far, every placed entry is
+ * followed (in a second copy of the list) by itself plus half the spacing.
+ * Any other ss runs 0 .. ss - 1.  (The library forms the same order as the
+ * m-bit reversal of the position; the two are checked against each other and
+ * against literal lists.) */
 void ctor_subset_order(int ss, int *sub)
 {
-    int bits = 0;
-    for (int i = 0; i < 32; i++) bits += (ss >> i) & 1;
-    if (bits == 1) {
-        int m1 = 0;
-        sub[m1++] = 0;
-        for (int i = ss, m2 = 1; i > 1; i /= 2, m2 *= 2)
-            for (int j = 0; j < m2; j++) sub[m1++] = sub[j] + i / 2;
-    } else {
+    if (ss & (ss - 1)) {
         for (int i = 0; i < ss; i++) sub[i] = i;
+        return;
+    }
+    int placed = 1;
+    sub[0] = 0;
+    for (int half = ss / 2; half >= 1; half /= 2) {
+        for (int j = 0; j < placed; j++) sub[placed + j] = sub[j] + half;
+        placed *= 2;
     }
 }
 

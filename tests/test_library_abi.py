@@ -61,3 +61,21 @@ def test_create_without_gpu_reports_no_device():
     lib = C.CDLL(LIB)
     h = C.c_void_p()
     assert lib.ctr_create(0, C.byref(h)) == 3   # CTR_NO_DEVICE
+
+
+def test_subset_order_matches_the_oracle_for_every_count():
+    """The library forms the order as a bit reversal, the oracle by doubling
+    the list: they agree for every subset count up to 256."""
+    import oracle as O
+    from ctrecon import subset_order
+    for ss in range(1, 257):
+        assert subset_order(ss).tolist() == O.subset_order(ss).tolist(), ss
+
+
+def test_query_layout_host_only():
+    from ctrecon._lib import query_layout
+    l = query_layout(1024, 1024, 1024)
+    assert l.pad == 100 and l.ipitch % 16 == 0 and l.ipitch >= 1224 and l.tpitch >= 1224
+    assert l.roff >= 1 and l.rpitch >= 1024 + 2 * l.roff and l.rpitch % 16 == 0
+    lib = C.CDLL(LIB)
+    assert lib.ctr_query_layout(0, 4, 4, None) == 1

@@ -605,3 +605,233 @@ def test_reslice_execution_composition():
     assert np.array_equal(ys, O.zcorrection(xp, 20, H - 20))
     want = O.denoise_sinogram(O.reslice(mt, None, ys, baseup=0.5, y0=100, n_layers=80), 1.0)
     assert np.array_equal(prj, want)
+
+
+# ---- independent pins of the nearest-bin rules, scales and offsets ----------
+#
+# Each expected value below is computed in FP64 numpy from the reference's
+# kernel text at one or two angles, with bins chosen independently of the
+# oracle: rounding floor(s + P/2 + 0.5) (OSEM.cl:95, AART.cl:97,
+# reconstShare.cl:265) against truncation floor(s + P/2) (FISTA.cl:237, :101),
+# integer halves X - Nx/2 (reconstShare.cl:265, FISTA.cl:101) against float
+# halves X - Nx*0.5f.  Sizes are odd so the two kinds of halves differ, and
+# only pixels whose continuous bin coordinate lies at least 0.02 from an
+# integer are compared, so FP32 and FP64 pick the same bin there.
+
+def _probe_bins(nx, ny, px, theta_deg, round_half, int_half):
+    """Continuous bin coordinate and its bin for every pixel at one angle."""
+    th = np.float64(np.float32(np.float32(theta_deg) * np.float64(np.pi) / 180.0))
+    c, s = np.cos(th), np.sin(th)
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny))
+    if int_half:
+        xr, yr, ph = X - nx // 2, Y - ny // 2, float(px // 2)
+    else:
+        xr, yr, ph = X - nx * 0.5, Y - ny * 0.5, px * 0.5
+    t = xr * c - yr * s + ph + (0.5 if round_half else 0.0)
+    return t, np.floor(t).astype(np.int64)
+
+
+def _probe_rows(pa, px):
+    # distinct value per (angle, bin): a wrong bin or a wrong row changes it
+    return (1.0 + np.arange(px)[None, :] + 100.0 * np.arange(pa)[:, None])[None]
+
+
+def _gather(rows, th, b, px):
+    ok = (b >= 0) & (b < px)
+    return np.where(ok, rows[0, th][np.clip(b, 0, px - 1)], 0.0)
+
+
+def _safe(t):
+    f = t - np.floor(t)
+    return (f > 0.02) & (f < 0.98)
+
+
+PROBE_ANGLES = (75.5225, 21.3, 128.7)       # cos 0.25 and two generic angles
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_nearest_bin_probes_round_vs_truncate(dtype):
+    """Single-angle subsets (pa = ss = 3, subset 1 = angle 1): each
+    backprojector's output against its closed form; pixels at bin
+    coordinate k + 0.5..1 separate rounding from truncation, and the odd
+    sizes separate integer from float halves."""
+    nx, ny, px, pa = 15, 13, 21, 3
+    D = max(nx, ny)
+    g = O.geom(nx, px, pa, ss=pa, ny=ny)
+    ang = np.array(PROBE_ANGLES, np.float32)
+    rows = _probe_rows(pa, px)
+    rng = np.random.default_rng(31)
+    x0 = rng.uniform(0.5, 1.5, (1, ny, nx))
+    alpha, L = 0.7, np.array([0.9, 0.3, 2.0])
+    sub = 1
+    cases = {
+        # name: (oracle output, round_half, int_half, closed form from the gathered sum q)
+        "osem2": (O.osem2(g, ang, x0, rows, sub, dtype=dtype), True, False,
+                  lambda q: x0[0] * q * pa / pa),
+        "aart2": (O.aart2(g, ang, x0, rows, sub, alpha, dtype=dtype), True, False,
+                  lambda q: x0[0] + q * alpha * pa / pa / D),
+        "backprojection": (O.backprojection(g, ang, rows, sub, dtype=dtype), True, True,
+                           lambda q: q * pa / pa / D),
+        "fista_backprojection": (O.fista_backprojection(g, ang, x0, rows, L, sub, alpha, dtype=dtype),
+                                 False, False, lambda q: x0[0] + q * alpha * pa / pa / D / L[sub] * 0.5),
+    }
+    for name, (out, rh, ih, form) in cases.items():
+        t, b = _probe_bins(nx, ny, px, ang[sub], rh, ih)
+        q = _gather(rows, sub, b, px)
+        m = _safe(t)
+        assert m.sum() > 0.8 * m.size, name
+        want = form(q)
+        assert np.allclose(out[0][m], want[m], rtol=2e-6, atol=1e-6), name
+        # the test discriminates: the other rounding rule and the other kind
+        # of halves each predict different values at many compared pixels
+        for rh2, ih2 in ((not rh, ih), (rh, not ih)):
+            t2, b2 = _probe_bins(nx, ny, px, ang[sub], rh2, ih2)
+            alt = form(_gather(rows, sub, b2, px))
+            m2 = m & _safe(t2)
+            assert (np.abs(alt[m2] - want[m2]) > 1e-3).sum() > 0.1 * m2.sum(), (name, rh2, ih2)
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+def test_power_iter2_probe_truncates_with_integer_halves(dtype):
+    """powerIter2 (FISTA.cl:79-110): layer Z gathers its compact row i of
+    angle Z + i*SS at floor((X - Nx/2)cos - (Y - Ny/2)sin + P/2), integer
+    halves, no +0.5, then * SS / A / D."""
+    nx, ny, px, pa = 15, 13, 21, 3
+    D = max(nx, ny)
+    g = O.geom(nx, px, pa, ss=pa, ny=ny)
+    ang = np.array(PROBE_ANGLES, np.float32)
+    prjc = np.zeros((pa, pa, px))
+    for z in range(pa):
+        prjc[z, 0] = 1.0 + np.arange(px) + 100.0 * z    # layer z: compact row 0 = angle z
+    out = O.power_iter2(g, ang, prjc, dtype=dtype)
+    for z in range(pa):
+        t, b = _probe_bins(nx, ny, px, ang[z], False, True)
+        ok = (b >= 0) & (b < px)
+        want = np.where(ok, prjc[z, 0][np.clip(b, 0, px - 1)], 0.0) * pa / pa / D
+        m = _safe(t)
+        assert np.allclose(out[z][m], want[m], rtol=2e-6), z
+        t2, b2 = _probe_bins(nx, ny, px, ang[z], True, True)
+        ok2 = (b2 >= 0) & (b2 < px)
+        alt = np.where(ok2, prjc[z, 0][np.clip(b2, 0, px - 1)], 0.0) * pa / pa / D
+        m2 = m & _safe(t2)
+        assert (np.abs(alt[m2] - want[m2]) > 1e-4).sum() > 0.1 * m2.sum()
+
+
+def test_update_scales_over_a_two_angle_subset():
+    """The update constants with SS != A: pa = 6, ss = 3, subset 2 = angles
+    2 and 5.  Rows all equal to 1 make the gathered sum the count of the
+    subset's angles whose bin falls on the detector (2 in the centre), so
+    OSEM2 multiplies by 2 * 3/6 = 1, AART2 adds 2 * alpha * 3/6/D,
+    backProjection gives 2 * 3/6/D and FISTAbackProjection adds
+    2 * alpha * 3/6/D/L[2] * 0.5."""
+    n, px, pa, ss, sub = 17, 17, 6, 3, 2
+    g = O.geom(n, px, pa, ss=ss)
+    ang = np.array([0, 30, 60, 90, 120, 150], np.float32)
+    ones = np.ones((1, pa, px))
+    x0 = np.full((1, n, n), 0.75)
+    alpha, L = 0.6, np.array([1.0, 1.0, 0.25])
+    yy, xx = np.mgrid[0:n, 0:n]
+    centre = (xx - n / 2) ** 2 + (yy - n / 2) ** 2 < (n / 2 - 2) ** 2
+    got = {
+        "osem2": O.osem2(g, ang, x0, ones, sub, dtype=F64)[0],
+        "aart2": O.aart2(g, ang, x0, ones, sub, alpha, dtype=F64)[0],
+        "backprojection": O.backprojection(g, ang, ones, sub, dtype=F64)[0],
+        "fista_backprojection": O.fista_backprojection(g, ang, x0, ones, L, sub, alpha, dtype=F64)[0],
+    }
+    want = {
+        "osem2": 0.75 * 1.0,
+        "aart2": 0.75 + 2 * 0.6 * 0.5 / 17,
+        "backprojection": 2 * 0.5 / 17,
+        "fista_backprojection": 0.75 + 2 * 0.6 * 0.5 / 17 / 0.25 * 0.5,
+    }
+    for k in got:
+        assert np.allclose(got[k][centre], want[k], rtol=1e-12), k
+
+
+# ---- powerIter1: the geometry without the +0.5 terms ------------------------
+
+def test_power_iter1_half_texel_closed_forms():
+    """powerIter1 (FISTA.cl:66-72) samples at x = (X - P/2)cos + (Y - D/2)sin +
+    Nx/2 with integer halves and no +0.5: at theta = 0 every sample sits on a
+    texel corner, so each ray is the half-texel average of columns X - 1 and
+    X (column -1 reads the border 0), p = (C[X-1] + C[X]) / 2 when every row
+    is hit twice (here Ny < D).  OSEM1 at the same angle samples texel
+    centres in x: p = C[X], the plain column sums.  At theta = 90 degrees
+    powerIter1 averages rows 13 - X and 14 - X over columns -1..15, so the
+    last column is counted once: p = (R[13-X] + R[14-X]) / 2 - (T[13-X, 15] +
+    T[14-X, 15]) / 4."""
+    nx, ny, px = 16, 12, 16
+    rng = np.random.default_rng(32)
+    img = rng.uniform(0.0, 1.0, (1, ny, nx))
+    g = O.geom(nx, px, 2, ss=1, ny=ny)
+    ang = np.array([0.0, 90.0], np.float32)
+    p1 = O.power_iter1(g, ang, img, dtype=F64)[0]
+    C = img[0].sum(axis=0)
+    Cm1 = np.concatenate([[0.0], C[:-1]])
+    assert np.allclose(p1[0], 0.5 * (Cm1 + C), rtol=1e-12)
+    R = img[0].sum(axis=1)
+    T = img[0]
+    X = np.arange(px)
+    r0, r1 = 13 - X, 14 - X
+
+    def row_or0(v, r):
+        return np.where((r >= 0) & (r < ny), v[np.clip(r, 0, ny - 1)], 0.0)
+    want90 = 0.5 * (row_or0(R, r0) + row_or0(R, r1)) - 0.25 * (row_or0(T[:, 15], r0) + row_or0(T[:, 15], r1))
+    assert np.allclose(p1[1], want90, rtol=1e-5, atol=1e-6)      # cos(float 90 deg) = -4.4e-8
+    # OSEM1 at theta = 0 on the same grid: column sums (texel centres)
+    prj = np.ones((1, 2, px))
+    r = O.osem1(g, ang, img, prj, 0, dtype=F64)
+    assert np.allclose(1.0 / r[0, 0], C, rtol=1e-12)
+    assert not np.allclose(p1[0], C, rtol=1e-3)
+
+
+# ---- FISTA3D momentum: literal values ----------------------------------------
+
+def test_fista3d_momentum_literals():
+    """FISTA.cl:584-592 with lambda = 0 (the prox returns v itself, x_new = v):
+    beta' = (sqrt(4 beta^2 + 1) + 1) / 2, gamma = (beta - 1) / beta',
+    w = gamma x_new + (1 - gamma) x.  Literal values: beta = 0 -> beta' = 1,
+    gamma = -1, w = 2x - v; beta = 1 -> beta' = 1.6180339887 (the golden
+    ratio), gamma = 0, w = x; beta = 3 -> beta' = 3.5413812651,
+    gamma = 0.5647513923; beta = 10 -> beta' = 10.5124921973, gamma = 0.8561242978."""
+    n = 4
+    g = O.geom(n, n, 1, ss=1)
+    betas = np.array([0.0, 1.0, 3.0, 10.0])
+    b = np.repeat(betas[:, None, None], n * n, axis=1).reshape(4, n, n)
+    rng = np.random.default_rng(33)
+    x = rng.uniform(0.5, 1.0, (4, n, n))
+    v = rng.uniform(0.5, 1.0, (4, n, n))
+    L = np.array([1.0])
+    for dtype, tol in ((F64, 1e-12), (F32, 2e-7)):
+        w, xn, bn = O.fista3d(g, x, v, b, L, 0, 0.0, dtype=dtype)
+        assert np.allclose(xn, v.astype(dtype), rtol=0, atol=0)
+        assert np.allclose(bn[:, 0, 0], [1.0, 1.6180339887498949, 3.5413812651491097, 10.512492197250394],
+                           rtol=tol)
+        gam = np.array([-1.0, 0.0, 2.0 / 3.5413812651491097, 9.0 / 10.512492197250394])
+        assert abs(gam[2] - 0.5647513923) < 1e-9 and abs(gam[3] - 0.8561242978) < 1e-9
+        want = gam[:, None, None] * v + (1.0 - gam[:, None, None]) * x
+        assert np.allclose(w, want, rtol=4 * tol, atol=4 * tol)
+        assert np.allclose(w[0], 2 * x[0] - v[0], rtol=4 * tol) and np.allclose(w[1], x[1], rtol=4 * tol)
+
+
+# ---- backProjectionFBP: where an impulse lands ------------------------------
+
+def test_fbp_backprojection_impulse_lands_on_its_line():
+    """backProjectionFBP (FBP.cl:181-186) samples the filtered row linearly at
+    u = s + P/2 + 0.5, i.e. at continuous bin s + P/2 with bin centres at
+    integers.  With one angle and an impulse at bin b0, theta = 0 lights
+    exactly column X = b0 (weight 1, value 1/(2A)) inside the disk; theta =
+    90 degrees lights exactly row Y = (Ny + P)/2 - b0.  A half-bin offset
+    error would split each line over two columns (rows) at 1/2."""
+    n, px = 32, 32
+    g = O.geom(n, px, 1, ss=1)
+    b0 = 11
+    fprj = np.zeros((1, 1, px))
+    fprj[0, 0, b0] = 1.0
+    yy, xx = np.mgrid[0:n, 0:n]
+    disk = np.hypot(xx - n / 2, yy - n / 2) < n / 2
+    for theta, lit in ((0.0, xx == b0), (90.0, yy == (n + px) // 2 - b0)):
+        out = O.fbp_backprojection(g, np.array([theta], np.float32), fprj, dtype=F64)[0]
+        want = np.where(lit & disk, 1.0 / 2.0, 0.0)
+        assert np.allclose(out, want, atol=1e-6), theta
+        assert np.abs(out[lit & disk] - 0.5).max() < 1e-6
