@@ -1291,6 +1291,18 @@ extern "C" ctr_status ctr_op_fista3d(ctr_ctx *c, const float *x, const float *v,
     return CTR_OK;
 }
 
+// The per-slice TV prox of the joint alignment model (no reference
+// counterpart; the 2-D form of ISTA3D, FISTA.cl:409-493).
+extern "C" ctr_status ctr_tv_prox_2d(ctr_ctx *c, const float *v, float *x_new, int dN, float lambda, float lower)
+{
+    TRY(op_prologue(c, v, x_new, dN));
+    if (!(lambda >= 0.0f) || !std::isfinite(lambda)) return fail(c, CTR_INVALID_ARGUMENT, "lambda must be finite and >= 0");
+    if (v == x_new) return fail(c, CTR_INVALID_ARGUMENT, "the TV prox needs v != x_new");
+    launch_tv2d(v, x_new, dN, c->g.nx, c->g.ny, lambda, lower, c->stream);
+    POST(c, "TV prox 2-D");
+    return CTR_OK;
+}
+
 // ===========================================================================
 // batched uneven stacks
 

@@ -86,6 +86,8 @@ void launch_sino_correction(const float *src, float *dst, const float2 *cs_corr,
 void launch_tv(int fista, const float *x, const float *v, const float *b, float *w_new, float *x_new,
                float *b_new, int out_padded, int ipitch, int nz, int nx, int ny, const float *L,
                int sub, double lambda_fista, hipStream_t stream);
+void launch_tv2d(const float *v, float *x_new, int nz, int nx, int ny, float lambda, float lower,
+                 hipStream_t stream);
 void launch_l2(const float *src, int nz, size_t n, float *partial, float *l2, hipStream_t stream);
 void launch_layer_min(const float *src, int nz, size_t n, float *partial, float *mins, hipStream_t stream);
 void launch_baseup(float *img, const float *mins, int nz, size_t plane, int order, hipStream_t stream);

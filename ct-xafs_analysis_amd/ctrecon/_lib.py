@@ -85,6 +85,7 @@ _SIGS = {
     "ctr_op_fista_backprojection": [_P, _P, _P, _P, _P, _I, _I, _F],
     "ctr_op_ista3d": [_P, _P, _P, _I, _I, _P],
     "ctr_op_fista3d": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P],
+    "ctr_tv_prox_2d": [_P, _P, _P, _I, _F, _F],
     "ctr_osem_batched": [_P, C.POINTER(Slice), _I, _I, _I],
     "ctr_autotune": [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "ctr_set_schedule": [_P, _I, _I, _I, _I],
@@ -515,6 +516,16 @@ class Context:
         f = getattr(self.lib, "ctr_op_" + name)
         conv = [(_ptr(a) if (a is None or isinstance(a, self._torch.Tensor)) else a) for a in args]
         self._run(f, (self.h, *conv), "ctr_op_" + name)
+
+    def tv_prox_2d(self, v, x_new, lam, lower=0.0):
+        """Per-slice TV prox with four in-plane neighbours (ctr_tv_prox_2d):
+        x_new = median of the neighbours and v - 4 lam .. v + 4 lam, then
+        max(lower, x).  v, x_new: distinct [dN][ny][nx] stacks."""
+        dN = _stack(v, "tv_prox_2d v")
+        self._img(v, dN, "tv_prox_2d v")
+        self._img(x_new, dN, "tv_prox_2d x_new")
+        self._run(self.lib.ctr_tv_prox_2d, (self.h, _ptr(v), _ptr(x_new), dN, float(lam), float(lower)),
+                  "ctr_tv_prox_2d")
 
     # ---- batched ----
     def osem_batched(self, imgs, prjs, it, use_graph=True):

@@ -7,10 +7,12 @@ moved detector coordinates).  Each iteration:
 
   * one FISTA step on x: the gradient A^T (A z - y) and the offset gradient
     come from one backward pass through AlignedProjection (the library's
-    adjoint and shift-gradient kernels); the proximal step is the
-    reference's median-form TV prox ISTA3D (CT_reconstruction/FISTA.cl:409-493,
-    which also keeps x >= 1e-6), applied by the library with
-    lambda = tv * step.  With precondition=True the data term is
+    adjoint and shift-gradient kernels); the proximal step is the per-slice
+    median-form TV prox with four in-plane neighbours (ctr_tv_prox_2d, the
+    2-D form of CT_reconstruction/FISTA.cl:409-493), with lambda = tv * step
+    and x >= 0.  The slices of a stack are unrelated, so no slice is coupled
+    to another (the reference's ISTA3D would tie each slice to the next one
+    and to layer 0).  With precondition=True the data term is
     weighted by R = 1 / (A 1) (inverse ray lengths) and the step by
     C = 1 / (A^T 1) (the SIRT scaling), with which a unit step is stable;
     the TV weight is then applied with the mean of C as its step;
@@ -71,8 +73,7 @@ class _Group:
         else:
             self.rw = self.cw = None
             self.step = 1.0 / _norm2_sq(lib, nz, g.ny, g.nx, off, dev, power_iters)
-        # ISTA3D applies lambda = cs_lambda / L2norm[sub] * 0.5; this L2norm makes it tv * step
-        self.l2 = torch.tensor([0.5 * float(g.cs_lambda) / (tv * self.step)], dtype=torch.float32, device=dev)
+        self.lam = tv * self.step
         self.x = (torch.zeros(self.shape, dtype=torch.float32, device=dev) if x0 is None
                   else x0.to(dev, torch.float32).contiguous().clone())
         self.z = self.x.clone()
@@ -111,7 +112,7 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
                 data += float(0.5 * (r * r).sum())
                 v = (G.z - (G.step * zr.grad if G.cw is None else G.cw * zr.grad)).contiguous()
                 x_new = torch.empty_like(v)
-                G.lib.op("ista3d", v, x_new, G.nz, 0, G.l2)
+                G.lib.tv_prox_2d(v, x_new, G.lam, 0.0)
                 new_x.append(x_new)
                 if o.requires_grad:
                     pd = p.detach()

@@ -287,6 +287,15 @@ ctr_status ctr_op_fista3d(ctr_ctx *ctx, const float *x, const float *v, const fl
                           float *w_new, float *x_new, float *b_new, int dN, int sub,
                           const float *L2norm);
 
+/* Per-slice TV prox: x_new = argmin 1/2 (x - v)^2 + lambda sum |x - n_i| over
+ * the four in-plane neighbours n_i of each pixel (texels outside the slice
+ * read 0), i.e. the median of n_0..n_3 and v - 4 lambda .. v + 4 lambda in
+ * steps of 2 lambda; then max(lower, x), NaN -> lower.  v, x_new: device
+ * [dN][ny][nx], distinct buffers.  No reference counterpart: the 2-D form of
+ * ISTA3D (FISTA.cl:409-493) for the joint alignment model, whose slices are
+ * unrelated, so unlike ISTA3D no slice is coupled to another. */
+ctr_status ctr_tv_prox_2d(ctr_ctx *ctx, const float *v, float *x_new, int dN, float lambda, float lower);
+
 /* ---- batched uneven stacks (no reference counterpart: the reference runs one
  *      layer batch of equal size per OSEM_execution call) ------------------- */
 

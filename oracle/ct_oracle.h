@@ -68,6 +68,8 @@ void ctor_ista3d_##S(const ctor_geom *g, const R *v, R *x_new, int nz, int sub, 
 void ctor_fista3d_##S(const ctor_geom *g, const R *x, const R *v, const R *b,           \
                     R *w_new, R *x_new, R *b_new, int nz, int sub, const R *L,           \
                     double lambda_fista);                                                \
+void ctor_tv_prox2d_##S(const ctor_geom *g, const R *v, R *x_new, int nz, R lambda,      \
+                    R lower);                                                            \
 void ctor_osem_execution_##S(const ctor_geom *g, const float *ang, const int *sub,      \
                     R *img, R *prj, int nz, int it, int prj_correction,                  \
                     int correction_mode);                                                \

@@ -517,6 +517,35 @@ void S(ctor_ista3d)(const ctor_geom *g, const R *v, R *x_new, int nz, int sub,
             }
 }
 
+/* Per-slice TV prox of the joint alignment model (the 2-D form of ISTA3D's
+ * median prox, FISTA.cl:409-493; no reference kernel): the median of the four
+ * in-plane neighbours (border 0) and v - 4 lam, v - 2 lam, v, v + 2 lam,
+ * v + 4 lam, found here by sorting all nine values; then max(lower, x),
+ * NaN -> lower. */
+void S(ctor_tv_prox2d)(const ctor_geom *g, const R *v, R *x_new, int nz, R lambda, R lower)
+{
+    const int nx = g->nx, ny = g->ny;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for (int Z = 0; Z < nz; Z++)
+        for (int Y = 0; Y < ny; Y++)
+            for (int X = 0; X < nx; X++) {
+                const R *lz = v + (size_t)Z * nx * ny;
+                const R vi = lz[(size_t)Y * nx + X];
+                R t[9] = {S(texel)(lz, nx, ny, X + 1, Y), S(texel)(lz, nx, ny, X - 1, Y),
+                          S(texel)(lz, nx, ny, X, Y + 1), S(texel)(lz, nx, ny, X, Y - 1),
+                          vi - (R)4 * lambda, vi - (R)2 * lambda, vi, vi + (R)2 * lambda, vi + (R)4 * lambda};
+                for (int a = 1; a < 9; a++) {          /* insertion sort, ascending */
+                    R key = t[a];
+                    int b = a - 1;
+                    while (b >= 0 && t[b] > key) { t[b + 1] = t[b]; b--; }
+                    t[b + 1] = key;
+                }
+                R x = t[4];
+                x = (x > lower) ? x : lower;
+                x_new[((size_t)Z * ny + Y) * nx + X] = x;
+            }
+}
+
 /* FISTA3D, FISTA.cl:497-594: prox (no clamp), beta' = (sqrt(4 beta^2 + 1) + 1)/2,
  * gamma = (beta - 1)/beta', w' = gamma x_new + (1 - gamma) x (NaN -> 1e-6).
  * The x_new write happens before the beta write, so when both outputs are the

@@ -177,6 +177,15 @@ def fista3d(g, x, v, b, L, sub, lam, dtype=np.float32):
     return w, xn, bn
 
 
+def tv_prox2d(g, v, lam, lower=0.0, dtype=np.float32):
+    """Per-slice TV prox (four in-plane neighbours) of the joint alignment model."""
+    v = _arr(v, dtype)
+    out = np.zeros_like(v)
+    R = C.c_float if np.dtype(dtype) == np.float32 else C.c_double
+    _call("tv_prox2d", dtype, C.byref(g), _p(v), _p(out), C.c_int(v.shape[0]), R(lam), R(lower))
+    return out
+
+
 # ---- drivers ---------------------------------------------------------------
 
 def osem_execution(g, ang, img, prj, it, sub=None, prj_correction=True, correction_mode=0, dtype=np.float32):

@@ -835,3 +835,48 @@ def test_fbp_backprojection_impulse_lands_on_its_line():
         want = np.where(lit & disk, 1.0 / 2.0, 0.0)
         assert np.allclose(out, want, atol=1e-6), theta
         assert np.abs(out[lit & disk] - 0.5).max() < 1e-6
+
+
+# ---- per-slice TV prox of the joint alignment model -------------------------
+
+def test_tv_prox2d_is_the_l1_prox_with_four_neighbours():
+    """argmin_x 1/2 (x - v)^2 + lam sum_i |x - n_i| over the four in-plane
+    neighbours (border 0), by brute force on a fine grid, then max(lower, x)."""
+    rng = np.random.default_rng(34)
+    n = 7
+    g = O.geom(n, n, 1, ss=1)
+    v = rng.uniform(0.0, 1.0, (1, n, n))
+    v[0, 3, 3] = -0.2
+    lam = 0.04
+    grid = np.linspace(-0.6, 1.6, 220001)
+    for dtype in (F64, F32):
+        out = O.tv_prox2d(g, v, lam, lower=-1.0, dtype=dtype)
+        for (yy, xx) in [(2, 3), (0, 0), (6, 6), (3, 3), (0, 4), (5, 1)]:
+            nb = [v[0, yy, xx + 1] if xx + 1 < n else 0.0, v[0, yy, xx - 1] if xx >= 1 else 0.0,
+                  v[0, yy + 1, xx] if yy + 1 < n else 0.0, v[0, yy - 1, xx] if yy >= 1 else 0.0]
+            f = 0.5 * (grid - v[0, yy, xx]) ** 2 + lam * sum(np.abs(grid - t) for t in nb)
+            assert abs(grid[np.argmin(f)] - out[0, yy, xx]) < 2e-5, (dtype, yy, xx)
+    # the floor
+    out0 = O.tv_prox2d(g, v, lam, lower=0.0, dtype=F64)
+    assert out0.min() >= 0.0 and out0[0, 3, 3] == 0.0
+
+
+def test_tv_prox2d_slices_are_independent():
+    """Each slice's result depends on that slice alone: changing the other
+    slices changes nothing (unlike ISTA3D, whose z-neighbours couple every
+    slice to the next one and to layer 0)."""
+    rng = np.random.default_rng(35)
+    g = O.geom(9, 9, 1, ss=1)
+    v = rng.uniform(0.0, 1.0, (4, 9, 9))
+    a = O.tv_prox2d(g, v, 0.03, dtype=F64)
+    w = v.copy()
+    w[[0, 1, 3]] = rng.uniform(0.0, 1.0, (3, 9, 9))
+    b = O.tv_prox2d(g, w, 0.03, dtype=F64)
+    assert np.array_equal(a[2], b[2])
+    single = O.tv_prox2d(g, v[2:3], 0.03, dtype=F64)
+    assert np.array_equal(a[2], single[0])
+    # ISTA3D over the same stack does couple slice 2 to the others
+    L = np.array([0.5])
+    i3a = O.ista3d(g, v, L, 0, 0.03, dtype=F64)
+    i3b = O.ista3d(g, w, L, 0, 0.03, dtype=F64)
+    assert not np.array_equal(i3a[2], i3b[2])
