@@ -28,19 +28,24 @@ Printed beside it:
   per_slice_launch   the same kernels launched one slice at a time with a
                      synchronisation after every kernel, as GPU.cpp does
                      (finish() after each enqueue, GPU.cpp:216-227)
-  relerr_*           one slice-iteration of slice 0 against the CPU oracle's
-                     FP32 compat restatement and its FP64 version
+  relerr_*           slice 0 after all the run's iterations (relerr_iterations)
+                     against the CPU oracle's FP32 compat restatement and its
+                     FP64 version, from the same start
   roofline           the forward-projection kernel (the dominant one): its
                      algorithmic FLOPs per launch over its mean launch time,
                      measured with HIP events on the library stream
   cpu_baseline       the oracle's FP32 restatement (OpenMP, all host threads)
-                     timed on one slice-iteration of a 1024^2 slice
+                     timed on one slice-iteration of a 1024^2 slice: 1 warm-up
+                     and 5 timed runs, the median, with the thread count,
+                     nproc and the CPU model
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ct-xafs_analysis_amd"))
@@ -84,27 +89,53 @@ def make_batch(B, n, pa, seed, device):
     return ang, sinos
 
 
-def cpu_baseline(n, pa, ss, ang, sino0, gpu_slice0, n_time, sino_time):
-    """Oracle FP32 compat and FP64 on one slice-iteration of slice 0 (the
-    parity sample), and the FP32 oracle timed on one n_time^2 slice-iteration."""
-    import numpy as np
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pa, ss, ang, n_time, sino_time, reps=5):
+    """The oracle's FP32 restatement (OpenMP, all host threads) timed on one
+    OS-EM slice-iteration of an n_time^2 slice: 1 warm-up run, then `reps`
+    timed runs; the median is reported."""
+    import statistics
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    gt = O.geom(n_time, n_time, pa, ss=ss)
+    xt = np.ones((1, n_time, n_time), np.float32)
+    O.osem_execution(gt, ang, xt.copy(), sino_time[None].copy(), 1)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.osem_execution(gt, ang, xt.copy(), sino_time[None].copy(), 1)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {
+        "value": 1.0 / med, "unit": "slice-iters/s", "cores": O.num_threads(), "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(), "kind": "port",
+        "sample": f"1 OS-EM slice-iteration ({ss} subsets) of one {n_time}^2 x {pa}-angle slice, FP32 compat "
+                  f"restatement of OSEM1/OSEM2 in C with OpenMP; 1 warm-up + {reps} runs, median {med:.3f} s "
+                  f"(runs {', '.join(f'{t:.3f}' for t in times)} s)",
+    }
+
+
+def parity_vs_oracle(n, pa, ss, ang, sino0, gpu_slice0, iters):
+    """Slice 0 after the run's `iters` OS-EM iterations against the oracle's
+    FP32 compat restatement and its FP64 version, from the same start."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     x0 = np.ones((1, n, n), np.float32)
     g = O.geom(n, n, pa, ss=ss)
-    c32 = O.osem_execution(g, ang, x0, sino0[None], 1)
-    c64 = O.osem_execution(g, ang, x0, sino0[None], 1, dtype=np.float64)
-    gt = O.geom(n_time, n_time, pa, ss=ss)
-    xt = np.ones((1, n_time, n_time), np.float32)
-    t0 = time.perf_counter()
-    O.osem_execution(gt, ang, xt, sino_time[None], 1)
-    t32 = time.perf_counter() - t0
+    c32 = O.osem_execution(g, ang, x0.copy(), sino0[None].copy(), iters)
+    c64 = O.osem_execution(g, ang, x0.copy(), sino0[None].copy(), iters, dtype=np.float64)
     rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))  # noqa: E731
-    return {
-        "value": 1.0 / t32, "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
-        "sample": f"1 OS-EM slice-iteration ({ss} subsets) of one {n_time}^2 x {pa}-angle slice, FP32 compat "
-                  f"restatement of OSEM1/OSEM2 in C with OpenMP; {t32:.2f} s",
-    }, rel(gpu_slice0, c32), rel(gpu_slice0, c64)
+    return rel(gpu_slice0, c32), rel(gpu_slice0, c64)
 
 
 def reslice_main(args):
@@ -335,15 +366,16 @@ def main():
         tune = ctx.autotune(nmax)     # a batched launch runs the schedule of its largest side
         log("autotune (fp rays/wave, fp waves/block, bp rows/thread):", tune)
 
-    # first call: one iteration from the uniform start -> parity sample of slice 0
+    # first call: one iteration from the uniform start
     ctx.osem_batched(imgs, sinos, 1, use_graph=True)
     ctx.synchronize()
-    gpu_slice0 = imgs[0].cpu().numpy()[None]
+    iters_done = 1
 
     # kernel timing pass (eager launches, HIP events on the library stream)
     ctx.set_kernel_timing(True)
     ctx.osem_batched(imgs, sinos, 1, use_graph=False)
     ctx.synchronize()
+    iters_done += 1
     fp_ms, bp_ms, nfp, nbp = ctx.kernel_timing()
     ctx.set_kernel_timing(False)
     log(f"FP kernel {fp_ms:.3f} ms/launch, BP kernel {bp_ms:.3f} ms/launch ({nfp}, {nbp} launches)")
@@ -369,6 +401,8 @@ def main():
     ctx.synchronize()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    iters_done += args.warmup + args.steps
+    gpu_slice0 = imgs[0].cpu().numpy()[None]        # after every iteration of the run
     value = eq * args.steps / dt
     log(f"batched: {value:.2f} (1024^2-equivalent) slice-iters/s ({dt / args.steps * 1e3:.1f} ms per step)")
 
@@ -439,16 +473,18 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu:
-        log("CPU baseline / oracle check on slice 0 ...")
+        log(f"oracle check of slice 0 after the run's {iters_done} iterations ...")
+        e32, e64 = parity_vs_oracle(sides[0], pa, ss, ang, sinos[0].cpu().numpy(), gpu_slice0, iters_done)
+        log(f"relerr vs FP32 compat {e32:.3e}, vs FP64 {e64:.3e}; CPU baseline ...")
         i_time = sides.index(1024) if 1024 in sides else 0
-        cb, e32, e64 = cpu_baseline(sides[0], pa, ss, ang, sinos[0].cpu().numpy(), gpu_slice0,
-                                    sides[i_time], sinos[i_time].cpu().numpy())
+        cb = cpu_baseline(pa, ss, ang, sides[i_time], sinos[i_time].cpu().numpy())
         if args.workload == "mixed":
             cb["value"] *= sides[i_time] ** 2 / 1024.0 ** 2
             cb["unit"] = unit
         out["cpu_baseline"] = cb
         out["relerr_vs_fp32_compat"] = e32
         out["relerr_vs_fp64"] = e64
+        out["relerr_iterations"] = iters_done
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -13,7 +13,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -54,12 +56,52 @@ struct DevBuf {
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // A set of slices with device descriptors and logical-block maps per grid kind.
+// The descriptors and the six maps sit in one device block (dev, or a region
+// the caller provides), d_sl / d_map pointing into it.
 struct SliceSet {
     std::vector<SliceDev> h;
     std::vector<int> hmap[6];
-    DevBuf dsl, dmap[6];
+    DevBuf dev;
+    const SliceDev *d_sl = nullptr;
+    const int *d_map[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int nb[6] = {0, 0, 0, 0, 0, 0};
     int fp_wpb = 4, fp_rays = 64, bp_rows = 4, na = 1;
+};
+
+// Pinned host staging for host -> device uploads (slice descriptors, block
+// maps, shift tables).  An upload copies its bytes into the next region of a
+// pinned ring and queues an asynchronous copy from there on the context
+// stream, so a call returns before its uploads complete.  A region is reused
+// only after the event recorded behind the copy that read it has completed;
+// regions are handed out in ring order, so the spans still in flight that a
+// new region can overlap are the oldest ones.
+struct Staging {
+    struct Span {
+        size_t off, len;
+        hipEvent_t ev;
+    };
+    char *host = nullptr;
+    size_t cap = 0, head = 0;
+    std::deque<Span> busy;
+    std::vector<hipEvent_t> idle;
+    ~Staging() { release(); }
+    void drain()
+    {
+        for (Span &sp : busy) {
+            (void)hipEventSynchronize(sp.ev);
+            idle.push_back(sp.ev);
+        }
+        busy.clear();
+    }
+    void release()
+    {
+        drain();
+        for (hipEvent_t e : idle) (void)hipEventDestroy(e);
+        idle.clear();
+        if (host) (void)hipHostFree(host);
+        host = nullptr;
+        cap = head = 0;
+    }
 };
 
 struct Layout {
@@ -119,11 +161,12 @@ struct ctr_ctx {
     DevBuf w_pimg, w_pimgT, w_rows, w_pimg2, w_pimgT2, w_aux1, w_aux2, w_aux3, w_partial, w_l2, w_tmpfull, w_min;
     DevBuf w_rcs, w_shift;   // rotation-centre search: candidate sinograms, candidate shifts
     DevBuf w_xproj, w_stage; // sinogram formation: x-projections, sinograms before spike removal
-    DevBuf w_gpad;           // alignment adjoint: projection rows with zero margins
+    DevBuf w_gpad;           // alignment adjoint: slice-fastest projection rows with zero margins
     SliceSet set_a, set_b;
     // autotune table: n -> schedule
     std::map<int, Tune> tune;
     std::unique_ptr<BatchPlan> plan;
+    Staging stage;
     // kernel timing
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -194,11 +237,77 @@ Tune tuned(ctr_ctx *c, int n)
     return t;
 }
 
-// Fills `S` with one descriptor per slice and uploads descriptors and maps.
-ctr_status upload_set(ctr_ctx *c, SliceSet &S)
+// Copies the parts (host pointer, bytes) back to back, each at a 256-byte
+// aligned offset, into a pinned staging region and queues one asynchronous
+// copy of the region to dst on the context stream.  `may_grow` = 0 forbids
+// allocating (a larger ring or a new event): the upload then waits for
+// earlier ones instead, or fails when it cannot fit.
+struct Part {
+    const void *src;
+    size_t bytes;
+};
+
+size_t parts_bytes(const Part *parts, int n)
 {
-    // the previous upload from these host vectors must have completed
-    HIPTRY(c, hipStreamSynchronize(c->stream));
+    size_t t = 0;
+    for (int i = 0; i < n; ++i) t = ((t + 255) & ~(size_t)255) + parts[i].bytes;
+    return t;
+}
+
+ctr_status stage_upload(ctr_ctx *c, void *dst, const Part *parts, int n, bool may_grow)
+{
+    Staging &st = c->stage;
+    const size_t total = std::max(parts_bytes(parts, n), (size_t)16);
+    if (total > st.cap) {
+        if (!may_grow)
+            return fail(c, CTR_INVALID_ARGUMENT, "upload of %zu bytes exceeds the %zu-byte staging ring", total,
+                        st.cap);
+        st.drain();
+        if (st.host) HIPTRY(c, hipHostFree(st.host));
+        st.host = nullptr;
+        st.cap = st.head = 0;
+        const size_t want = std::max(total, (size_t)4 << 20);
+        HIPTRY(c, hipHostMalloc((void **)&st.host, want, hipHostMallocDefault));
+        st.cap = want;
+    }
+    size_t off = st.head;
+    if (off + total > st.cap) off = 0;
+    while (!st.busy.empty()) {
+        const Staging::Span &f = st.busy.front();
+        if (f.off >= off + total || f.off + f.len <= off) break;       // the oldest span is clear of the region
+        HIPTRY(c, hipEventSynchronize(f.ev));
+        st.idle.push_back(f.ev);
+        st.busy.pop_front();
+    }
+    size_t o = 0;
+    for (int i = 0; i < n; ++i) {
+        o = (o + 255) & ~(size_t)255;
+        if (parts[i].bytes) std::memcpy(st.host + off + o, parts[i].src, parts[i].bytes);
+        o += parts[i].bytes;
+    }
+    HIPTRY(c, hipMemcpyAsync(dst, st.host + off, total, hipMemcpyHostToDevice, c->stream));
+    hipEvent_t ev = nullptr;
+    if (!st.idle.empty()) {
+        ev = st.idle.back();
+        st.idle.pop_back();
+    } else if (may_grow) {
+        HIPTRY(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    } else {
+        // reuse the oldest span's event once its copy has completed
+        if (st.busy.empty()) return fail(c, CTR_HIP_ERROR, "no staging event available");
+        HIPTRY(c, hipEventSynchronize(st.busy.front().ev));
+        ev = st.busy.front().ev;
+        st.busy.pop_front();
+    }
+    HIPTRY(c, hipEventRecord(ev, c->stream));
+    st.busy.push_back({off, total, ev});
+    st.head = off + total;
+    return CTR_OK;
+}
+
+// Builds the per-kind logical-block maps of S (host only).
+void plan_maps(SliceSet &S)
+{
     const int n = (int)S.h.size();
     for (int k = 0; k < 6; ++k) S.hmap[k].clear();
     for (int s = 0; s < n; ++s) {
@@ -219,54 +328,91 @@ ctr_status upload_set(ctr_ctx *c, SliceSet &S)
     // segment x runs on XCD x (xcd_remap), and the segments are padded to one
     // length with empty entries.  Entries are (slice, block within the slice)
     // pairs; slice -1 is empty.
-    {
-        constexpr double BLOCK_COST_SAMPLES = 93.0;
-        std::vector<int> cnt(n);
-        double total = 0.0;
-        for (int s = 0; s < n; ++s) {
-            const SliceDev &d = S.h[s];
-            cnt[s] = fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays);
-            total += (double)cnt[s] * (std::max(d.nx, d.ny) + BLOCK_COST_SAMPLES);
-        }
-        std::vector<std::vector<std::pair<int, int>>> seg(8);
-        double acc = 0.0;
-        int x = 0;
-        for (int s = 0; s < n; ++s) {
-            const double w = std::max(S.h[s].nx, S.h[s].ny) + BLOCK_COST_SAMPLES;
-            for (int b = 0; b < cnt[s]; ++b) {
-                while (x < 7 && acc + 0.5 * w > total * (x + 1) / 8.0) ++x;
-                seg[x].push_back({s, b});
-                acc += w;
-            }
-        }
-        size_t per = 0;
-        for (const auto &v : seg) per = std::max(per, v.size());
-        std::vector<int> &fm = S.hmap[G_FP];
-        fm.assign(2 * 8 * per, -1);
-        for (int k = 0; k < 8; ++k)
-            for (size_t i = 0; i < seg[k].size(); ++i) {
-                fm[2 * (k * per + i)] = seg[k][i].first;
-                fm[2 * (k * per + i) + 1] = seg[k][i].second;
-            }
+    constexpr double BLOCK_COST_SAMPLES = 93.0;
+    std::vector<int> cnt(n);
+    double total = 0.0;
+    for (int s = 0; s < n; ++s) {
+        const SliceDev &d = S.h[s];
+        cnt[s] = fp_blocks(d.px, S.na, S.fp_wpb, S.fp_rays);
+        total += (double)cnt[s] * (std::max(d.nx, d.ny) + BLOCK_COST_SAMPLES);
     }
-    HIPTRY(c, S.dsl.ensure(sizeof(SliceDev) * std::max(n, 1)));
-    HIPTRY(c, hipMemcpyAsync(S.dsl.p, S.h.data(), sizeof(SliceDev) * n, hipMemcpyHostToDevice, c->stream));
+    std::vector<std::vector<std::pair<int, int>>> seg(8);
+    double acc = 0.0;
+    int x = 0;
+    for (int s = 0; s < n; ++s) {
+        const double w = std::max(S.h[s].nx, S.h[s].ny) + BLOCK_COST_SAMPLES;
+        for (int b = 0; b < cnt[s]; ++b) {
+            while (x < 7 && acc + 0.5 * w > total * (x + 1) / 8.0) ++x;
+            seg[x].push_back({s, b});
+            acc += w;
+        }
+    }
+    size_t per = 0;
+    for (const auto &v : seg) per = std::max(per, v.size());
+    std::vector<int> &fm = S.hmap[G_FP];
+    fm.assign(2 * 8 * per, -1);
+    for (int k = 0; k < 8; ++k)
+        for (size_t i = 0; i < seg[k].size(); ++i) {
+            fm[2 * (k * per + i)] = seg[k][i].first;
+            fm[2 * (k * per + i) + 1] = seg[k][i].second;
+        }
     for (int k = 0; k < 6; ++k) {
         const size_t entries = S.hmap[k].size();
         S.nb[k] = (int)(k == G_FP ? entries / 2 : entries);
-        HIPTRY(c, S.dmap[k].ensure(sizeof(int) * std::max(entries, (size_t)2)));
-        if (entries)
-            HIPTRY(c, hipMemcpyAsync(S.dmap[k].p, S.hmap[k].data(), sizeof(int) * entries, hipMemcpyHostToDevice,
-                                     c->stream));
     }
+}
+
+// The parts of S's device block: descriptors, then the six maps.
+int set_parts(const SliceSet &S, Part *parts)
+{
+    parts[0] = {S.h.data(), sizeof(SliceDev) * S.h.size()};
+    for (int k = 0; k < 6; ++k) parts[1 + k] = {S.hmap[k].data(), sizeof(int) * S.hmap[k].size()};
+    return 7;
+}
+
+// Points d_sl / d_map into the device block at `base`, laid out as parts_bytes lays it out.
+void bind_set(SliceSet &S, char *base, const Part *parts)
+{
+    size_t o = 0;
+    for (int i = 0; i < 7; ++i) {
+        o = (o + 255) & ~(size_t)255;
+        if (i == 0) S.d_sl = reinterpret_cast<const SliceDev *>(base + o);
+        else S.d_map[i - 1] = reinterpret_cast<const int *>(base + o);
+        o += parts[i].bytes;
+    }
+}
+
+// Plans the maps of `S` and uploads descriptors and maps into its device
+// block (grown when needed) or, when `dst` is given, into that region (which
+// must hold set_bytes(S) bytes).  Asynchronous: the host vectors can change
+// as soon as this returns.
+size_t set_bytes(SliceSet &S)
+{
+    plan_maps(S);
+    Part parts[7];
+    return parts_bytes(parts, set_parts(S, parts));
+}
+
+ctr_status upload_set(ctr_ctx *c, SliceSet &S, char *dst = nullptr)
+{
+    plan_maps(S);
+    Part parts[7];
+    const int np = set_parts(S, parts);
+    const size_t bytes = parts_bytes(parts, np);
+    if (!dst) {
+        HIPTRY(c, S.dev.ensure(bytes));
+        dst = static_cast<char *>(S.dev.p);
+    }
+    TRY(stage_upload(c, dst, parts, np, dst == S.dev.p));
+    bind_set(S, dst, parts);
     return CTR_OK;
 }
 
 Launch launch_of(ctr_ctx *c, SliceSet &S, int kind)
 {
     Launch L;
-    L.sl = static_cast<const SliceDev *>(S.dsl.p);
-    L.bmap = static_cast<const int *>(S.dmap[kind].p);
+    L.sl = S.d_sl;
+    L.bmap = S.d_map[kind];
     L.cs = static_cast<const float2 *>(c->cs.p);
     L.nblocks = S.nb[kind];
     return L;
@@ -287,7 +433,17 @@ struct StackBind {
     int layer_base = 0;
 };
 
+void fill_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b);
+
 ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
+{
+    fill_stack(c, S, nz, b);
+    return upload_set(c, S);
+}
+
+// The descriptors of a uniform stack of nz layers of the context geometry
+// (host only).
+void fill_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
 {
     const ctr_geometry &g = c->g;
     const Layout l = make_layout(g.nx, g.ny, g.px);
@@ -311,7 +467,6 @@ ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
         d.aux = b.aux ? b.aux + (size_t)z * g.nx * g.ny : nullptr;
         d.layer = b.layer_base + z;
     }
-    return upload_set(c, S);
 }
 
 // Device workspace for an nz-layer stack of the current geometry.
@@ -477,6 +632,20 @@ extern "C" ctr_status ctr_create(int device, ctr_ctx **out)
     }
     c->stream = c->own;
     for (auto &e : c->ev) (void)hipEventCreate(&e);
+    // the staging ring and its events exist from the start
+    constexpr size_t RING = (size_t)4 << 20;
+    if (hipHostMalloc((void **)&c->stage.host, RING, hipHostMallocDefault) != hipSuccess) {
+        c->stage.host = nullptr;
+        (void)hipStreamDestroy(c->own);
+        for (auto &e : c->ev) (void)hipEventDestroy(e);
+        delete c;
+        return CTR_OUT_OF_MEMORY;
+    }
+    c->stage.cap = RING;
+    for (int i = 0; i < 16; ++i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) c->stage.idle.push_back(e);
+    }
     *out = c;
     return CTR_OK;
 }
@@ -619,6 +788,22 @@ static ctr_status require_full_grid(ctr_ctx *c)
     return CTR_OK;
 }
 
+// The OS-EM iterations over a bound stack: pad the images in, build the
+// transposed copy, then per subset OSEM1 -> OSEM2 (each update also refreshes
+// the transposed copy the next projection reads), and unpad the result.
+static ctr_status osem_iterations(ctr_ctx *c, SliceSet &S, int it)
+{
+    TRY(run_kind(c, S, G_PAD));
+    TRY(run_kind(c, S, G_TR));
+    for (int i = 0; i < it; ++i)
+        for (int k = 0; k < c->g.ss; ++k) {
+            TRY(run_fp(c, S, FP_RATIO, c->sub[k], 0, 0));
+            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
+        }
+    TRY(run_kind(c, S, G_EW));
+    return CTR_OK;
+}
+
 // OSEM_execution, GPU.cpp:125-240: [sinogramCorrection] then for it, for k:
 // OSEM1(sub[k]) -> OSEM2(sub[k]) -> image update.
 extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int dN, int it, int prj_correction)
@@ -630,23 +815,14 @@ extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int
     TRY(ensure_ws(c, dN, false, 0));
     TRY(zero_rows(c, dN, g.pa / g.ss));
     if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
     b.prj = prj; b.src = img; b.dst = img;
     SliceSet &S = c->set_a;
     TRY(build_stack(c, S, dN, b));
-    TRY(run_kind(c, S, G_PAD));
-    TRY(run_kind(c, S, G_TR));
-    // each update also refreshes the transposed copy the next projection reads
-    for (int i = 0; i < it; ++i)
-        for (int k = 0; k < g.ss; ++k) {
-            TRY(run_fp(c, S, FP_RATIO, c->sub[k], 0, 0));
-            TRY(run_bp(c, S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
-        }
-    TRY(run_kind(c, S, G_EW));
-    return CTR_OK;
+    return osem_iterations(c, S, it);
 }
+
 
 // FISTA_OSEM_execution, GPU.cpp:243-405.  L2norm is filled with 0.5
 // (GPU.cpp:326), so lambda = CSlambda.  First sub-iteration: ISTA3D of the
@@ -667,7 +843,6 @@ extern "C" ctr_status ctr_fista_osem_execution(ctr_ctx *c, float *img, float *pr
     launch_fill(c->w_l2.f(), g.ss, 0.5f, c->stream);
     POST(c, "fill L2norm");
     if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     const Layout l = make_layout(g.nx, g.ny, g.px);
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
@@ -705,7 +880,6 @@ extern "C" ctr_status ctr_aart_execution(ctr_ctx *c, float *img, float *prj, int
     TRY(ensure_ws(c, dN, false, 0));
     TRY(zero_rows(c, dN, g.pa / g.ss));
     if (g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     const float alpha = (g.mode == 3) ? g.wt2 : g.wt1;   // GPU.cpp:1161-1165
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
@@ -922,9 +1096,8 @@ ctr_status upload_shifts(ctr_ctx *c, int n, float start, float step)
     std::vector<float> h((size_t)n);
     for (int i = 0; i < n; ++i) h[i] = start + step * (float)i;
     HIPTRY(c, c->w_shift.ensure(sizeof(float) * (size_t)n));
-    HIPTRY(c, hipMemcpyAsync(c->w_shift.p, h.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
-    return CTR_OK;
+    const Part part{h.data(), sizeof(float) * (size_t)n};
+    return stage_upload(c, c->w_shift.p, &part, 1, true);
 }
 
 // setMask's float2 angles {a*PI/180, a*PI/180 - PI_2} are formed in double
@@ -1012,7 +1185,6 @@ extern "C" ctr_status ctr_power_iteration(ctr_ctx *c, float *L2norm, int trials)
     const size_t plane = (size_t)g.nx * g.ny;
     launch_fill(c->w_aux2.f(), plane * S, 1.0f, c->stream);
     POST(c, "fill ones");
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     const Layout l = make_layout(g.nx, g.ny, g.px);
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
@@ -1052,7 +1224,6 @@ extern "C" ctr_status ctr_fista_execution(ctr_ctx *c, const float *L2norm, float
     const size_t plane = (size_t)g.nx * g.ny * dN;
     launch_fill(c->w_aux2.f(), plane, 0.0f, c->stream);   // beta = 0 (GPU.cpp:1799)
     POST(c, "fill beta");
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     // set_a: R (x) in pimg; set_b: W in pimg2.  aux1 = v, aux2 = beta.
     StackBind br;
     br.pimg = c->w_pimg.f(); br.pimgT = c->w_pimgT.f(); br.rows = c->w_rows.f();
@@ -1115,7 +1286,6 @@ ctr_status op_stack(ctr_ctx *c, int dN, const float *img, const float *prj, floa
 {
     TRY(ensure_ws(c, dN, false, 0));
     TRY(zero_rows(c, dN, std::max(c->g.pa / c->g.ss, 1)));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f();
     b.prj = prj; b.src = img; b.aux = aux; b.layer_base = layer_base;
@@ -1387,7 +1557,6 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
             orr += (size_t)na * l.rpitch;
         }
         TRY(upload_set(c, P.set));
-        HIPTRY(c, hipStreamSynchronize(c->stream));
     }
     BatchPlan &P = *c->plan;
     TRY(run_kind(c, P.set, G_PAD));
@@ -1549,7 +1718,6 @@ ctr_status align_pad(ctr_ctx *c, const float *img, int dN)
 {
     const ctr_geometry &g = c->g;
     TRY(ensure_ws(c, dN, false, 0));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
     StackBind b;
     b.pimg = c->w_pimg.f(); b.pimgT = c->w_pimgT.f(); b.rows = c->w_rows.f(); b.src = img;
     TRY(build_stack(c, c->set_a, dN, b));
@@ -1581,7 +1749,7 @@ extern "C" ctr_status ctr_align_adjoint(ctr_ctx *c, const float *prj, const floa
     TRY(op_prologue(c, prj, img, dN));
     if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
     const ctr_geometry &g = c->g;
-    HIPTRY(c, c->w_gpad.ensure(sizeof(float) * (size_t)dN * g.pa * align_adj_pitch(g.px)));
+    HIPTRY(c, c->w_gpad.ensure(sizeof(float) * align_adj_floats(dN, g.pa, g.px)));
     launch_align_adjoint(prj, c->w_gpad.f(), static_cast<const float2 *>(c->cs.p), shifts, center, img, dN, g.nx,
                          g.ny, g.pa, g.px, c->stream);
     POST(c, "align adjoint");

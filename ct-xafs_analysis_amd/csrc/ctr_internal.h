@@ -123,10 +123,11 @@ void launch_full_to_rows(const float *full, float *rows, int nz, int na, int rpi
 void launch_align_forward(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,
                           const float *shifts, float center, float *prj, int nz, int nx, int ny, int pa,
                           int px, hipStream_t stream);
-// Zero floats on either side of each projection row in the adjoint's copy of
-// the rows, and that copy's row pitch.
+// Zero rays on either side of each projection row in the adjoint's copy of
+// the rows, and that copy's size in floats (slice-fastest, in chunks of 4 or
+// 16 slices).
 constexpr int AD_PAD = 2;
-int align_adj_pitch(int px);
+size_t align_adj_floats(int nz, int pa, int px);
 void launch_align_adjoint(const float *prj, float *gpad, const float2 *cs, const float *shifts, float center,
                           float *img, int nz, int nx, int ny, int pa, int px, hipStream_t stream);
 void launch_align_grad(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,

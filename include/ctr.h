@@ -13,9 +13,16 @@
  *  - Image stacks are [dN][ny][nx] float32, sinogram stacks [dN][pa][px]
  *    float32, row-major with x fastest: the layouts of the reference's
  *    Image2DArray objects (GPU.cpp:422-423) and of its raw files.
- *  - Calls are asynchronous on the context stream unless stated otherwise;
- *    ctr_synchronize() waits.  A context is thread-compatible, not
- *    thread-safe: use one context per host thread.
+ *  - Calls are asynchronous on the context stream: the execution drivers and
+ *    operators queue their uploads (through a pinned staging ring) and
+ *    kernels and return; ctr_synchronize() waits.  These block the host:
+ *    ctr_set_geometry and ctr_destroy (they wait for the context's queued
+ *    work), ctr_autotune (it times kernels), ctr_osem_batched when its slice
+ *    list differs from the previous call's (it rebuilds its plan), any call
+ *    that must grow a workspace buffer (the old one is freed, which waits
+ *    for the device), ctr_set_sync_each_kernel(1) mode, and kernel timing.
+ *    A context is thread-compatible, not thread-safe: use one context per
+ *    host thread.
  *  - Errors are returned as ctr_status; ctr_last_error() gives the message.
  *    The library never exits or aborts the caller's process.
  */

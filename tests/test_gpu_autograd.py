@@ -127,7 +127,7 @@ def test_joint_alignment_recovery(ctx):
           f" -> {hist[-1]:.3e}", flush=True)
     assert e1 < 0.3 and e1 < 0.2 * e0
     assert hist[-1] < 0.2 * h0[-1]
-    assert float(x.min()) >= 1e-6 - 1e-12
+    assert float(x.min()) >= 0.0          # the per-slice TV prox keeps x >= 0
 
 
 def test_joint_alignment_shared_offsets_uneven_stack():

@@ -52,8 +52,10 @@ def test_osem_512sq_900proj_20it(ctx):
     ctx.osem_execution(x, dev(sino), it)
     got = host(x)
     c32 = O.osem_execution(g, ang, x0, sino, it)
-    e32 = relerr(got, c32)
-    print(f"[parity] OS-EM 512^2/900/S10/it20: relerr vs FP32 compat {e32:.3e}", flush=True)
+    c64 = O.osem_execution(g, ang, x0, sino, it, dtype=np.float64)
+    e32, e64 = relerr(got, c32), relerr(got, c64)
+    print(f"[parity] OS-EM 512^2/900/S10/it20: relerr vs FP32 compat {e32:.3e}, vs FP64 {e64:.3e} "
+          f"(FP32 compat vs FP64 {relerr(c32, c64):.3e})", flush=True)
     assert e32 < 1e-4
 
 

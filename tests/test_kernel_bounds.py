@@ -179,3 +179,47 @@ def test_projector_samples_tap_inside_their_box(nx, ny, px):
         ic, ir = np.trunc(lc).astype(np.int64), np.trunc(lr).astype(np.int64)
         assert ic.min() >= 0 and ir.min() >= 0
         assert ic.max() + 1 < T["bw"][a, ch, ti] and ir.max() + 1 < T["bh"][a, ch, ti]
+
+
+AJ_T, AJ_W, AD_PAD = 16, 32, 2
+
+
+def _window_start(du0, dv0, c, s, pxh, sh, center, px):
+    """k_align_adj_lds's ceil(Xs - 1.45) clamped to [-AD_PAD, px - 1], in float32
+    with the kernel's operation order (no fused operations)."""
+    Xs = ((((du0 * c).astype(f32) - (dv0 * s).astype(f32)).astype(f32) + pxh).astype(f32) + sh).astype(f32)
+    Xs = (Xs + f32(center)).astype(f32)
+    return np.clip(np.ceil((Xs - f32(1.45)).astype(f32)), -AD_PAD, px - 1).astype(np.int64)
+
+
+@pytest.mark.parametrize("nx,ny,px", [(37, 37, 41), (64, 48, 20), (512, 512, 512), (2048, 2048, 2048), (40, 3900, 3900)])
+def test_adjoint_tiles_stage_every_window(nx, ny, px):
+    """Each thread of a 16 x 16 tile reads rays xa .. xa + 2 of the AJ_W rays
+    its block staged from sbase = clip(min over the tile's corners of xa,
+    -AD_PAD, px + AD_PAD - AJ_W): all inside [sbase, sbase + AJ_W) and inside
+    the padded row [-AD_PAD, px + AD_PAD)."""
+    rng = np.random.default_rng(nx * 3 + px)
+    C, S = _angles(px)
+    C, S = C[::3], S[::3]
+    pxh = f32(px * 0.5)
+    cu, cv = f32(nx * 0.5), f32(ny * 0.5)
+    tx_n, ty_n = (nx + AJ_T - 1) // AJ_T, (ny + AJ_T - 1) // AJ_T
+    tiles = [(tx, ty) for tx in range(tx_n) for ty in range(ty_n)]
+    if len(tiles) > 40:
+        tiles = [tiles[0], tiles[-1]] + [tiles[k] for k in rng.choice(len(tiles), 38, replace=False)]
+    for sh, center in ((0.0, 0.0), (6.3, -2.9), (-7.75, 3.0)):
+        for tx, ty in tiles:
+            i = np.minimum(tx * AJ_T + np.arange(AJ_T), nx - 1)
+            j = np.minimum(ty * AJ_T + np.arange(AJ_T), ny - 1)
+            du0 = (i.astype(f32) - cu).astype(f32)[None, :, None]
+            dv0 = (j.astype(f32) - cv).astype(f32)[None, None, :]
+            c, s = C[:, None, None], S[:, None, None]
+            xa = _window_start(du0, dv0, c, s, pxh, f32(sh), center, px)
+            corners = [_window_start(du0[:, [0, -1][p]:, :][:, :1], dv0[:, :, [0, -1][q]:][:, :, :1], c, s, pxh,
+                                     f32(sh), center, px) for p in (0, 1) for q in (0, 1)]
+            lo = np.minimum.reduce(corners)[:, 0, 0]
+            sbase = np.maximum(np.minimum(lo, px + AD_PAD - AJ_W), -AD_PAD)
+            off = xa - sbase[:, None, None]
+            assert off.min() >= 0, (tx, ty, sh)
+            assert (off + 2).max() <= AJ_W - 1, (tx, ty, sh, (off + 2).max())
+            assert (xa + 2).max() <= px + AD_PAD - 1
