@@ -144,6 +144,33 @@ struct BatchPlan {
     }
 };
 
+// Workspace of ctr_osem_execution_noalloc: a file-scope device array, one
+// copy per device that the library runs on, held until the process ends.  Host
+// code never uses its name as an address: the device address of each copy is
+// resolved once per device (hipGetSymbolAddress, with that device current, at
+// the first ctr_create on it) and kept in scratch_ptr.
+static __device__ __align__(256) char scratch[CTR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[MAX_DEVICES] = {};
+bool scratch_tried[MAX_DEVICES] = {};
+
+// Resolves the current device's copy of `scratch` (call with `device`
+// current).  Returns its address, or null if it could not be resolved.
+void *resolve_scratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_tried[device]) {
+        scratch_tried[device] = true;
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) scratch_ptr[device] = ptr;
+        else (void)hipGetLastError();
+    }
+    return scratch_ptr[device];
+}
+
 }  // namespace
 
 struct ctr_ctx {
@@ -167,6 +194,8 @@ struct ctr_ctx {
     std::map<int, Tune> tune;
     std::unique_ptr<BatchPlan> plan;
     Staging stage;
+    SliceSet set_na;         // the no-allocation driver's slice set (descriptors in the scratch array)
+    char *scratch = nullptr; // this device's copy of the scratch array (device address)
     // kernel timing
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -632,7 +661,9 @@ extern "C" ctr_status ctr_create(int device, ctr_ctx **out)
     }
     c->stream = c->own;
     for (auto &e : c->ev) (void)hipEventCreate(&e);
-    // the staging ring and its events exist from the start
+    c->scratch = static_cast<char *>(resolve_scratch(device));
+    // the staging ring and its events exist from the start, so uploads of up
+    // to 4 MiB (every call of the no-allocation driver) allocate nothing
     constexpr size_t RING = (size_t)4 << 20;
     if (hipHostMalloc((void **)&c->stage.host, RING, hipHostMallocDefault) != hipSuccess) {
         c->stage.host = nullptr;
@@ -823,6 +854,46 @@ extern "C" ctr_status ctr_osem_execution(ctr_ctx *c, float *img, float *prj, int
     return osem_iterations(c, S, it);
 }
 
+// ctr_osem_execution with its workspace in the device array `scratch`
+// (CTR_NOALLOC_SCRATCH_BYTES per device) instead of context buffers: the
+// slice descriptors and block maps, then the padded images, their transposed
+// copies and the subset rows of every layer.  Allocates nothing: the
+// descriptors go through the staging ring created with the context.
+extern "C" ctr_status ctr_osem_execution_noalloc(ctr_ctx *c, float *img, float *prj, int dN, int it,
+                                                 int prj_correction)
+{
+    TRY(check_stack_args(c, img, prj, dN));
+    TRY(require_full_grid(c));
+    if (it < 0) return fail(c, CTR_INVALID_ARGUMENT, "it must be >= 0 (got %d)", it);
+    if (!c->scratch) return fail(c, CTR_HIP_ERROR, "the scratch array's device address could not be resolved");
+    const ctr_geometry &g = c->g;
+    const Layout l = make_layout(g.nx, g.ny, g.px);
+    const int na = g.pa / g.ss;
+    const size_t per_layer = sizeof(float) * (pimg_floats(l) + pimgT_floats(l) + (size_t)na * l.rpitch);
+    // the descriptor block's size does not depend on the buffer addresses
+    SliceSet &S = c->set_na;
+    StackBind none;
+    fill_stack(c, S, dN, none);
+    const size_t desc = (set_bytes(S) + 255) & ~(size_t)255;
+    if (desc > CTR_NOALLOC_SCRATCH_BYTES || (size_t)dN > (CTR_NOALLOC_SCRATCH_BYTES - desc) / per_layer)
+        return fail(c, CTR_INVALID_ARGUMENT,
+                    "%d layers of %d x %d with %d bins need %zu bytes of scratch; the no-allocation driver has %zu",
+                    dN, g.nx, g.ny, g.px, desc + (size_t)dN * per_layer, (size_t)CTR_NOALLOC_SCRATCH_BYTES);
+    HIPTRY(c, hipSetDevice(c->device));
+    char *base = c->scratch + desc;
+    StackBind b;
+    b.pimg = reinterpret_cast<float *>(base);
+    b.pimgT = b.pimg + (size_t)dN * pimg_floats(l);
+    b.rows = b.pimgT + (size_t)dN * pimgT_floats(l);
+    b.prj = prj; b.src = img; b.dst = img;
+    fill_stack(c, S, dN, b);
+    // rows start at zero on every call (the array keeps the last call's contents)
+    launch_fill(b.rows, (size_t)dN * na * l.rpitch, 0.0f, c->stream);
+    POST(c, "fill rows");
+    if (prj_correction && g.correction_mode > 0) TRY(correct_in_place(c, prj, dN));
+    TRY(upload_set(c, S, c->scratch));
+    return osem_iterations(c, S, it);
+}
 
 // FISTA_OSEM_execution, GPU.cpp:243-405.  L2norm is filled with 0.5
 // (GPU.cpp:326), so lambda = CSlambda.  First sub-iteration: ISTA3D of the

@@ -67,6 +67,7 @@ _SIGS = {
     "ctr_subset_order": [_I, _P],
     "ctr_query_layout": [_I, _I, _I, C.POINTER(Layout)],
     "ctr_osem_execution": [_P, _P, _P, _I, _I, _I],
+    "ctr_osem_execution_noalloc": [_P, _P, _P, _I, _I, _I],
     "ctr_fista_osem_execution": [_P, _P, _P, _I, _I, _I],
     "ctr_aart_execution": [_P, _P, _P, _I, _I],
     "ctr_fbp_execution": [_P, _P, _P, _I, _I],
@@ -321,6 +322,13 @@ class Context:
         dN = self._driver_args(img, prj, "osem_execution")
         self._run(self.lib.ctr_osem_execution, (self.h, _ptr(img), _ptr(prj), dN, it, int(prj_correction),),
                   "ctr_osem_execution")
+
+    def osem_execution_noalloc(self, img, prj, it, prj_correction=True):
+        """ctr_osem_execution_noalloc: the same results with the workspace in
+        the library's per-device scratch array (no allocation)."""
+        dN = self._driver_args(img, prj, "osem_execution_noalloc")
+        self._run(self.lib.ctr_osem_execution_noalloc, (self.h, _ptr(img), _ptr(prj), dN, it, int(prj_correction),),
+                  "ctr_osem_execution_noalloc")
 
     def fista_osem_execution(self, img, prj, it, prj_correction=True):
         dN = self._driver_args(img, prj, "fista_osem_execution")

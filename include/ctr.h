@@ -121,6 +121,26 @@ ctr_status ctr_query_layout(int nx, int ny, int px, ctr_layout *out);
 ctr_status ctr_osem_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
                               int dN, int it, int prj_correction);
 
+/* Bytes of the per-device scratch array of ctr_osem_execution_noalloc. */
+#define CTR_NOALLOC_SCRATCH_BYTES ((unsigned long long)64 << 20)
+
+/* ctr_osem_execution that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No reference
+ * counterpart.  Its workspace is a file-scope device array of
+ * CTR_NOALLOC_SCRATCH_BYTES (64 MiB) that each device the library runs on
+ * holds until the process ends: per layer the padded image and its
+ * transposed copy, (nx + 200 rounded up to 16) x (ny + 200) floats each, and
+ * pa/ss subset rows of px + 2 roff floats (ctr_query_layout), plus a few KB
+ * of descriptors -- e.g. up to 5 layers of 1024^2 or 20 of 512^2.  A larger
+ * stack returns CTR_INVALID_ARGUMENT before anything runs.  The results equal
+ * ctr_osem_execution's bit for bit.  Every call on one device shares that
+ * device's array: calls that overlap on one device (two contexts, or two
+ * host threads) race and give wrong results; calls on one context are
+ * ordered by its stream. */
+ctr_status ctr_osem_execution_noalloc(ctr_ctx *ctx, float *reconst_img, float *prj_img,
+                                      int dN, int it, int prj_correction);
+
 /* FISTA_OSEM_execution, GPU.cpp:243-405 (CT_reconstruction.hpp:78-81). */
 ctr_status ctr_fista_osem_execution(ctr_ctx *ctx, float *reconst_img, float *prj_img,
                                     int dN, int it, int prj_correction);
