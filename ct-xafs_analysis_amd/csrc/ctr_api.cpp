@@ -65,7 +65,13 @@ struct SliceSet {
     const SliceDev *d_sl = nullptr;
     const int *d_map[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int nb[6] = {0, 0, 0, 0, 0, 0};
-    int fp_wpb = 4, fp_rays = 64, bp_rows = 4, na = 1;
+    int fp_wpb = 4, fp_rays = 64, fp_tile = 60, bp_rows = 4, na = 1;
+    // the maps in `dev` and what they were planned from (the schedule and
+    // every slice's extents); own_at is null when dev holds no valid maps
+    std::vector<int> map_sig;
+    const void *own_at = nullptr;
+    const int *own_map[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int own_nb[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Pinned host staging for host -> device uploads (slice descriptors, block
@@ -126,14 +132,26 @@ Layout make_layout(int nx, int ny, int px)
 size_t pimg_floats(const Layout &l) { return (size_t)(l.ny + 2 * PAD) * l.ipitch; }
 size_t pimgT_floats(const Layout &l) { return (size_t)(l.nx + 2 * PAD) * l.tpitch; }
 
+// A projector / backprojector schedule: the LDS-tiled projector (fp_rays = 0)
+// with tiles of fp_tile samples, or the direct-gather one with fp_rays rays
+// per wave and fp_wpb waves per block; bp_rows rows per backprojection thread.
 struct Tune {
-    int fp_wpb = 4, fp_rays = 0, bp_rows = 8;   // LDS-tiled projection, 8-row backprojection
+    int fp_wpb = 4, fp_rays = 0, fp_tile = 60, bp_rows = 8;   // the defaults
+    bool operator==(const Tune &o) const
+    {
+        return fp_wpb == o.fp_wpb && fp_rays == o.fp_rays && fp_tile == o.fp_tile && bp_rows == o.bp_rows;
+    }
 };
 
+// The LDS-tiled projector covers a ray of D samples with at most 64 tiles.
+bool tile_fits(int tile, int side) { return (long long)tile * 64 >= side; }
+
+// A batched stack: one slice set per distinct schedule (the slices of each
+// side run the schedule chosen for that side), all in one set of buffers.
 struct BatchPlan {
     std::vector<ctr_slice> key;
     int it_graph = -1;
-    SliceSet set;
+    std::vector<std::unique_ptr<SliceSet>> sets;
     DevBuf pimg, pimgT, rows;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -259,6 +277,7 @@ Tune tuned(ctr_ctx *c, int n)
 {
     auto it = c->tune.find(n);
     Tune t = it != c->tune.end() ? it->second : Tune{};
+    if (t.fp_rays == 0 && !tile_fits(t.fp_tile, n)) t.fp_tile = 60;
     if (t.fp_rays == 0 && n > LDS_FP_MAX_SIDE) {
         t.fp_rays = 64;
         t.fp_wpb = 2;
@@ -334,6 +353,33 @@ ctr_status stage_upload(ctr_ctx *c, void *dst, const Part *parts, int n, bool ma
     return CTR_OK;
 }
 
+// Logical blocks of slice d per grid kind (the projection's map is planned
+// separately).
+void block_counts(const SliceSet &S, const SliceDev &d, int cnt[6])
+{
+    cnt[0] = 0;
+    cnt[G_BP] = bp_blocks(d.nx, d.ny, S.bp_rows);
+    cnt[G_TR] = tr_blocks(d.nx, d.ny);
+    cnt[G_PAD] = pad_blocks(d.ny, d.ipitch);
+    cnt[G_EW] = ew_blocks(d.nx, d.ny);
+    cnt[G_SPARE] = 0;
+}
+
+// Sets each slice's first logical block per kind, as plan_maps does (host only).
+void block_offsets(SliceSet &S)
+{
+    int at[6] = {0, 0, 0, 0, 0, 0};
+    for (SliceDev &d : S.h) {
+        int cnt[6];
+        block_counts(S, d, cnt);
+        for (int k = 1; k < 6; ++k) {
+            d.blk[k] = at[k];
+            at[k] += cnt[k];
+        }
+        d.blk[G_FP] = 0;
+    }
+}
+
 // Builds the per-kind logical-block maps of S (host only).
 void plan_maps(SliceSet &S)
 {
@@ -341,8 +387,8 @@ void plan_maps(SliceSet &S)
     for (int k = 0; k < 6; ++k) S.hmap[k].clear();
     for (int s = 0; s < n; ++s) {
         SliceDev &d = S.h[s];
-        const int cnt[6] = {0, bp_blocks(d.nx, d.ny, S.bp_rows), tr_blocks(d.nx, d.ny), pad_blocks(d.ny, d.ipitch),
-                            ew_blocks(d.nx, d.ny), 0};
+        int cnt[6];
+        block_counts(S, d, cnt);
         for (int k = 1; k < 6; ++k) {
             d.blk[k] = (int)S.hmap[k].size();
             S.hmap[k].insert(S.hmap[k].end(), (size_t)cnt[k], s);
@@ -422,17 +468,54 @@ size_t set_bytes(SliceSet &S)
     return parts_bytes(parts, set_parts(S, parts));
 }
 
+// The inputs of plan_maps: the schedule and each slice's extents.
+void map_signature(const SliceSet &S, std::vector<int> &sig)
+{
+    sig.clear();
+    sig.reserve(5 + 4 * S.h.size());
+    sig.insert(sig.end(), {S.na, S.fp_wpb, S.fp_rays, S.fp_tile, S.bp_rows});
+    for (const SliceDev &d : S.h) sig.insert(sig.end(), {d.nx, d.ny, d.px, d.ipitch});
+}
+
 ctr_status upload_set(ctr_ctx *c, SliceSet &S, char *dst = nullptr)
 {
+    if (!dst) {
+        // A stack called again with the same extents and schedule (an
+        // iterative solver calling an operator every iteration) keeps the maps
+        // already in its block and uploads only the descriptors, which hold
+        // the buffer addresses: the maps of a large stack are megabytes.
+        std::vector<int> sig;
+        map_signature(S, sig);
+        if (S.own_at && S.own_at == S.dev.p && sig == S.map_sig) {
+            block_offsets(S);
+            const Part p0 = {S.h.data(), sizeof(SliceDev) * S.h.size()};
+            TRY(stage_upload(c, S.dev.p, &p0, 1, true));
+            S.d_sl = static_cast<const SliceDev *>(S.dev.p);
+            for (int k = 0; k < 6; ++k) {
+                S.d_map[k] = S.own_map[k];
+                S.nb[k] = S.own_nb[k];
+            }
+            return CTR_OK;
+        }
+        plan_maps(S);
+        Part parts[7];
+        const int np = set_parts(S, parts);
+        S.own_at = nullptr;
+        HIPTRY(c, S.dev.ensure(parts_bytes(parts, np)));
+        TRY(stage_upload(c, S.dev.p, parts, np, true));
+        bind_set(S, static_cast<char *>(S.dev.p), parts);
+        S.map_sig.swap(sig);
+        S.own_at = S.dev.p;
+        for (int k = 0; k < 6; ++k) {
+            S.own_map[k] = S.d_map[k];
+            S.own_nb[k] = S.nb[k];
+        }
+        return CTR_OK;
+    }
     plan_maps(S);
     Part parts[7];
     const int np = set_parts(S, parts);
-    const size_t bytes = parts_bytes(parts, np);
-    if (!dst) {
-        HIPTRY(c, S.dev.ensure(bytes));
-        dst = static_cast<char *>(S.dev.p);
-    }
-    TRY(stage_upload(c, dst, parts, np, dst == S.dev.p));
+    TRY(stage_upload(c, dst, parts, np, false));
     bind_set(S, dst, parts);
     return CTR_OK;
 }
@@ -480,6 +563,7 @@ void fill_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
     const Tune t = tuned(c, std::max(g.nx, g.ny));
     S.fp_wpb = t.fp_wpb;
     S.fp_rays = t.fp_rays;
+    S.fp_tile = t.fp_tile;
     S.bp_rows = t.bp_rows;
     S.na = na;
     S.h.assign(nz, SliceDev{});
@@ -563,7 +647,7 @@ ctr_status run_fp(ctr_ctx *c, SliceSet &S, int var, int sub, int sub_from_layer,
 {
     FpParams p{sub, c->g.ss, c->g.pa / c->g.ss, sub_from_layer, geo_power, 0};
     time_begin(c, 0);
-    launch_fp(var, S.fp_wpb, S.fp_rays, launch_of(c, S, G_FP), p, c->stream);
+    launch_fp(var, S.fp_wpb, S.fp_rays, S.fp_tile, launch_of(c, S, G_FP), p, c->stream);
     POST(c, "forward projection");
     time_end(c, 0);
     return CTR_OK;
@@ -1563,8 +1647,8 @@ bool same_key(const std::vector<ctr_slice> &a, const ctr_slice *b, int n)
 ctr_status batched_iteration(ctr_ctx *c, BatchPlan &P)
 {
     for (int k = 0; k < c->g.ss; ++k) {
-        TRY(run_fp(c, P.set, FP_RATIO, c->sub[k], 0, 0));
-        TRY(run_bp(c, P.set, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
+        for (auto &S : P.sets) TRY(run_fp(c, *S, FP_RATIO, c->sub[k], 0, 0));
+        for (auto &S : P.sets) TRY(run_bp(c, *S, BP_OSEM2, c->sub[k], 0, 0.0f, nullptr, 0, 1));
     }
     return CTR_OK;
 }
@@ -1602,18 +1686,30 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
         HIPTRY(c, P.rows.ensure(sizeof(float) * fr));
         launch_fill(P.rows.f(), fr, 0.0f, c->stream);
         POST(c, "fill rows");
-        const int n0 = slices[order[0]].n;
-        const Tune t = tuned(c, n0);
-        P.set.fp_wpb = t.fp_wpb;
-        P.set.fp_rays = t.fp_rays;
-        P.set.bp_rows = t.bp_rows;
-        P.set.na = na;
-        P.set.h.assign(count, SliceDev{});
+        // one set per distinct schedule, in the order of each schedule's largest side
+        std::vector<Tune> tunes;
+        std::vector<int> set_of(count);
+        for (int r = 0; r < count; ++r) {
+            const Tune t = tuned(c, slices[order[r]].n);
+            int k = 0;
+            while (k < (int)tunes.size() && !(tunes[k] == t)) ++k;
+            if (k == (int)tunes.size()) tunes.push_back(t);
+            set_of[r] = k;
+        }
+        for (const Tune &t : tunes) {
+            auto S = std::make_unique<SliceSet>();
+            S->fp_wpb = t.fp_wpb;
+            S->fp_rays = t.fp_rays;
+            S->fp_tile = t.fp_tile;
+            S->bp_rows = t.bp_rows;
+            S->na = na;
+            P.sets.push_back(std::move(S));
+        }
         size_t oi = 0, ot = 0, orr = 0;
         for (int r = 0; r < count; ++r) {
             const int i = order[r];
             const Layout &l = lay[i];
-            SliceDev &d = P.set.h[r];
+            SliceDev d{};
             d.nx = d.ny = d.px = slices[i].n;
             d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch; d.roff = l.roff;
             d.img = P.pimg.f() + oi;
@@ -1623,15 +1719,18 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
             d.src = slices[i].img;
             d.dst = slices[i].img;
             d.layer = r;
+            P.sets[set_of[r]]->h.push_back(d);
             oi += pimg_floats(l);
             ot += pimgT_floats(l);
             orr += (size_t)na * l.rpitch;
         }
-        TRY(upload_set(c, P.set));
+        for (auto &S : P.sets) TRY(upload_set(c, *S));
     }
     BatchPlan &P = *c->plan;
-    TRY(run_kind(c, P.set, G_PAD));
-    TRY(run_kind(c, P.set, G_TR));
+    for (auto &S : P.sets) {
+        TRY(run_kind(c, *S, G_PAD));
+        TRY(run_kind(c, *S, G_TR));
+    }
     const bool graph = use_graph && !c->sync_each && !c->timing;
     if (graph && it > 0 && !P.exec) {
         HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1653,7 +1752,7 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
             TRY(batched_iteration(c, P));
         }
     }
-    TRY(run_kind(c, P.set, G_EW));
+    for (auto &S : P.sets) TRY(run_kind(c, *S, G_EW));
     return CTR_OK;
 }
 
@@ -1679,55 +1778,94 @@ extern "C" ctr_status ctr_kernel_timing(ctr_ctx *c, double *fp_ms, double *bp_ms
     return CTR_OK;
 }
 
-extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, int fp_rays, int fp_waves, int bp_rows)
+namespace {
+
+bool schedule_ok(const ctr_schedule &s, int n)
 {
-    if (!c) return CTR_INVALID_ARGUMENT;
-    const bool ok = (fp_rays == 0 || fp_rays == 64 || fp_rays == 32 || fp_rays == 16 || fp_rays == 8) &&
-                    (fp_waves == 2 || fp_waves == 4) &&
-                    (bp_rows == 1 || bp_rows == 2 || bp_rows == 4 || bp_rows == 8 || bp_rows == 16) && n >= 2 &&
-                    !(fp_rays == 0 && n > LDS_FP_MAX_SIDE);
-    if (!ok)
-        return fail(c, CTR_INVALID_ARGUMENT, "schedule (n=%d, rays=%d, waves=%d, rows=%d) not supported", n, fp_rays,
-                    fp_waves, bp_rows);
+    const bool lds = s.fp_rays == 0;
+    return n >= 2 && n <= 16384 &&
+           (lds ? (s.fp_waves == 4 && (s.fp_tile == 32 || s.fp_tile == 44 || s.fp_tile == 60) &&
+                   tile_fits(s.fp_tile, n) && n <= LDS_FP_MAX_SIDE)
+                : ((s.fp_rays == 64 || s.fp_rays == 32 || s.fp_rays == 16 || s.fp_rays == 8) &&
+                   (s.fp_waves == 2 || s.fp_waves == 4))) &&
+           (s.bp_rows == 1 || s.bp_rows == 2 || s.bp_rows == 4 || s.bp_rows == 8 || s.bp_rows == 16);
+}
+
+void to_schedule(const Tune &t, ctr_schedule *s)
+{
+    s->fp_rays = t.fp_rays;
+    s->fp_waves = t.fp_wpb;
+    s->fp_tile = t.fp_tile;
+    s->bp_rows = t.bp_rows;
+}
+
+}  // namespace
+
+extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, const ctr_schedule *s)
+{
+    if (!c || !s) return CTR_INVALID_ARGUMENT;
+    if (!schedule_ok(*s, n))
+        return fail(c, CTR_INVALID_ARGUMENT, "schedule (n=%d, rays=%d, waves=%d, tile=%d, rows=%d) not supported", n,
+                    s->fp_rays, s->fp_waves, s->fp_tile, s->bp_rows);
     Tune t;
-    t.fp_rays = fp_rays;
-    t.fp_wpb = fp_waves;
-    t.bp_rows = bp_rows;
+    t.fp_rays = s->fp_rays;
+    t.fp_wpb = s->fp_waves;
+    t.fp_tile = s->fp_rays == 0 ? s->fp_tile : 60;
+    t.bp_rows = s->bp_rows;
     c->tune[n] = t;
     c->plan.reset();
     return CTR_OK;
 }
 
-extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_waves, int *bp_rows)
+extern "C" ctr_status ctr_get_schedule(ctr_ctx *c, int n, ctr_schedule *s)
+{
+    if (!c || !s || n < 2 || n > 16384) return CTR_INVALID_ARGUMENT;
+    *s = ctr_schedule{};
+    to_schedule(tuned(c, n), s);
+    return CTR_OK;
+}
+
+// Times every projector and backprojector schedule for side n on the device
+// and keeps the fastest: the LDS-tiled projector with tiles of 60, 44 and 32
+// samples (those whose 64 tiles cover the side), the direct-gather projector
+// with 64 / 32 / 16 / 8 rays per wave and 2 or 4 waves per block, then 2, 4,
+// 8 or 16 backprojection rows per thread.  Each candidate is timed on a
+// launch of batch size: copies of one slice adding up to about eight 1024^2
+// slices of work (128 copies of a 256^2 slice), which fills the device at any
+// side.  The reference fixes its work-group shape per compiled program
+// (GPU.cpp:169-172, 538-547).
+extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, ctr_schedule *best_out)
 {
     TRY(need_geom(c));
     if (n < 2 || n > 16384) return fail(c, CTR_INVALID_ARGUMENT, "side %d out of range", n);
     HIPTRY(c, hipSetDevice(c->device));
     const int na = c->g.pa / c->g.ss;
     const int reps = 5;
-    // several descriptors on one slice, so every schedule is timed on a launch
-    // of batch size: about eight 1024^2 slices of work (128 copies of a 256^2
-    // slice), which fills the device whatever the side
-    const int copies = (int)std::max(8LL, std::min(256LL, 8LL * 1024 * 1024 / ((long long)n * n)));
-    // a synthetic slice: disk image, ones sinogram
+    // Synthetic slices of ones, each with its own image, transposed image, rows
+    // and sinogram, enough of them (1 GiB, 2..4096 slices) that a launch streams
+    // its slices from HBM as a large batch does: copies that shared one image
+    // would run from the caches and favour the schedules that re-read it most.
+    // Each part starts on a 256-byte boundary, as in the workspaces.
     const Layout l = make_layout(n, n, n);
-    DevBuf pimg, pimgT, rows, prj;
-    HIPTRY(c, pimg.ensure(sizeof(float) * pimg_floats(l)));
-    HIPTRY(c, pimgT.ensure(sizeof(float) * pimgT_floats(l)));
-    HIPTRY(c, rows.ensure(sizeof(float) * (size_t)na * l.rpitch));
-    HIPTRY(c, prj.ensure(sizeof(float) * (size_t)c->g.pa * n));
-    launch_fill(pimg.f(), pimg_floats(l), 1.0f, c->stream);
-    launch_fill(pimgT.f(), pimgT_floats(l), 1.0f, c->stream);
-    launch_fill(rows.f(), (size_t)na * l.rpitch, 0.0f, c->stream);
-    launch_fill(prj.f(), (size_t)c->g.pa * n, 1.0f, c->stream);
+    auto up64 = [](size_t f) { return (f + 63) / 64 * 64; };
+    const size_t f_img = up64(pimg_floats(l)), f_imgT = up64(pimgT_floats(l)),
+                 f_rows = up64((size_t)na * l.rpitch), f_prj = up64((size_t)c->g.pa * n);
+    const size_t f_slice = f_img + f_imgT + f_rows + f_prj;
+    const size_t target = (size_t)1 << 30;
+    const int copies = (int)std::max<size_t>(2, std::min<size_t>(4096, (target + 4 * f_slice - 1) / (4 * f_slice)));
+    DevBuf mem;
+    HIPTRY(c, mem.ensure(sizeof(float) * f_slice * copies));
+    launch_fill(mem.f(), f_slice * copies, 1.0f, c->stream);
     POST(c, "autotune fill");
     SliceSet S;
     S.na = na;
     S.h.assign(copies, SliceDev{});
-    for (SliceDev &d : S.h) {
+    for (int k = 0; k < copies; ++k) {
+        SliceDev &d = S.h[k];
+        float *base = mem.f() + f_slice * k;
         d.nx = d.ny = d.px = n;
         d.ipitch = l.ipitch; d.tpitch = l.tpitch; d.rpitch = l.rpitch; d.roff = l.roff;
-        d.img = pimg.f(); d.imgT = pimgT.f(); d.rows = rows.f(); d.prj = prj.f();
+        d.img = base; d.imgT = base + f_img; d.rows = base + f_img + f_imgT; d.prj = base + f_img + f_imgT + f_rows;
     }
     hipEvent_t e0, e1;
     HIPTRY(c, hipEventCreate(&e0));
@@ -1742,30 +1880,55 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
         (void)hipEventElapsedTime(&ms, e0, e1);
         return ms / reps;
     };
-    Tune best_t;
-    double best = 1e30;
-    const int ray_opts[5] = {0, 64, 32, 16, 8};
-    const int wpb_opts[2] = {2, 4};
-    for (int rays : ray_opts)
-        for (int w : wpb_opts) {
-            if (rays == 0 && (w != 4 || n > LDS_FP_MAX_SIDE)) continue;   // the LDS-tiled kernel has a fixed block
-            S.fp_wpb = w;
-            S.fp_rays = rays;
-            S.bp_rows = 4;
-            TRY(upload_set(c, S));
-            FpParams p{0, c->g.ss, na, 0, 0, 0};
-            const double ms = timeit([&] { launch_fp(FP_RATIO, w, rays, launch_of(c, S, G_FP), p, c->stream); });
-            if (ms < best) { best = ms; best_t.fp_wpb = w; best_t.fp_rays = rays; }
+    Tune def;                                    // the default schedule for this side
+    if (n > LDS_FP_MAX_SIDE) {
+        def.fp_rays = 64;
+        def.fp_wpb = 2;
+    }
+    std::vector<Tune> fp_cands;
+    for (int tile : {60, 44, 32})
+        if (n <= LDS_FP_MAX_SIDE && tile_fits(tile, n)) {
+            Tune t;
+            t.fp_tile = tile;
+            fp_cands.push_back(t);
         }
+    for (int rays : {64, 32, 16, 8})
+        for (int w : {2, 4}) {
+            Tune t;
+            t.fp_rays = rays;
+            t.fp_wpb = w;
+            fp_cands.push_back(t);
+        }
+    Tune best_t = def;
+    double best = 1e30, def_fp = 0.0, def_bp = 0.0;
+    for (const Tune &t : fp_cands) {
+        S.fp_wpb = t.fp_wpb;
+        S.fp_rays = t.fp_rays;
+        S.fp_tile = t.fp_tile;
+        S.bp_rows = 4;
+        TRY(upload_set(c, S));
+        FpParams p{0, c->g.ss, na, 0, 0, 0};
+        const double ms = timeit([&] { launch_fp(FP_RATIO, t.fp_wpb, t.fp_rays, t.fp_tile, launch_of(c, S, G_FP), p,
+                                                 c->stream); });
+        if (t.fp_rays == def.fp_rays && t.fp_wpb == def.fp_wpb && t.fp_tile == def.fp_tile) def_fp = ms;
+        if (ms < best) {
+            best = ms;
+            best_t.fp_wpb = t.fp_wpb;
+            best_t.fp_rays = t.fp_rays;
+            best_t.fp_tile = t.fp_tile;
+        }
+    }
+    const double best_fp = best;
     best = 1e30;
-    const int ropts[4] = {2, 4, 8, 16};
-    for (int r : ropts) {
+    for (int r : {2, 4, 8, 16}) {
         S.fp_wpb = best_t.fp_wpb;
         S.fp_rays = best_t.fp_rays;
+        S.fp_tile = best_t.fp_tile;
         S.bp_rows = r;
         TRY(upload_set(c, S));
         BpParams p{0, c->g.ss, na, c->g.pa, 0, 0.0f, nullptr, 0, 0};
         const double ms = timeit([&] { launch_bp(BP_OSEM2, r, launch_of(c, S, G_BP), p, c->stream); });
+        if (r == def.bp_rows) def_bp = ms;
         if (ms < best) { best = ms; best_t.bp_rows = r; }
     }
     (void)hipEventDestroy(e0);
@@ -1774,9 +1937,13 @@ extern "C" ctr_status ctr_autotune(ctr_ctx *c, int n, int *fp_rays, int *fp_wave
     HIPTRY(c, hipStreamSynchronize(c->stream));
     c->tune[n] = best_t;
     c->plan.reset();   // plans pick up the new choice on rebuild
-    if (fp_rays) *fp_rays = best_t.fp_rays;
-    if (fp_waves) *fp_waves = best_t.fp_wpb;
-    if (bp_rows) *bp_rows = best_t.bp_rows;
+    if (best_out) {
+        to_schedule(best_t, best_out);
+        best_out->fp_ms = (float)best_fp;
+        best_out->bp_ms = (float)best;
+        best_out->fp_ms_default = (float)def_fp;
+        best_out->bp_ms_default = (float)def_bp;
+    }
     return CTR_OK;
 }
 

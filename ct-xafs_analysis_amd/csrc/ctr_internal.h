@@ -74,7 +74,7 @@ struct BpParams {
 };
 
 // ---- launchers (ctr_kernels.hip) ----
-void launch_fp(int var, int wpb, int rays, const Launch &L, const FpParams &p, hipStream_t stream);
+void launch_fp(int var, int wpb, int rays, int tile, const Launch &L, const FpParams &p, hipStream_t stream);
 void launch_bp(int var, int rows, const Launch &L, const BpParams &p, hipStream_t stream);
 void launch_transpose(const Launch &L, hipStream_t stream);
 void launch_pad_in(const Launch &L, hipStream_t stream);
@@ -124,8 +124,8 @@ void launch_align_forward(const float *pimg, const float *pimgT, int ipitch, int
                           const float *shifts, float center, float *prj, int nz, int nx, int ny, int pa,
                           int px, hipStream_t stream);
 // Zero rays on either side of each projection row in the adjoint's copy of
-// the rows, and that copy's size in floats (slice-fastest, in chunks of 4 or
-// 16 slices).
+// the rows, and that copy's size in floats (slice-fastest, in chunks of 1, 4
+// or 16 slices).
 constexpr int AD_PAD = 2;
 size_t align_adj_floats(int nz, int pa, int px);
 void launch_align_adjoint(const float *prj, float *gpad, const float2 *cs, const float *shifts, float center,

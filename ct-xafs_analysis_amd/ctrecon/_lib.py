@@ -38,6 +38,16 @@ class Slice(C.Structure):
     _fields_ = [("n", C.c_int), ("img", C.c_void_p), ("prj", C.c_void_p)]
 
 
+class Schedule(C.Structure):
+    """ctr_schedule (include/ctr.h)."""
+    _fields_ = [("fp_rays", C.c_int), ("fp_waves", C.c_int), ("fp_tile", C.c_int), ("bp_rows", C.c_int),
+                ("fp_ms", C.c_float), ("bp_ms", C.c_float), ("fp_ms_default", C.c_float),
+                ("bp_ms_default", C.c_float)]
+
+    def as_dict(self):
+        return {f: (round(getattr(self, f), 4) if t is C.c_float else getattr(self, f)) for f, t in self._fields_}
+
+
 class Layout(C.Structure):
     """ctr_layout (include/ctr.h)."""
     _fields_ = [("pad", C.c_int), ("ipitch", C.c_int), ("tpitch", C.c_int), ("rpitch", C.c_int), ("roff", C.c_int)]
@@ -88,8 +98,9 @@ _SIGS = {
     "ctr_op_fista3d": [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P],
     "ctr_tv_prox_2d": [_P, _P, _P, _I, _F, _F],
     "ctr_osem_batched": [_P, C.POINTER(Slice), _I, _I, _I],
-    "ctr_autotune": [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
-    "ctr_set_schedule": [_P, _I, _I, _I, _I],
+    "ctr_autotune": [_P, _I, C.POINTER(Schedule)],
+    "ctr_set_schedule": [_P, _I, C.POINTER(Schedule)],
+    "ctr_get_schedule": [_P, _I, C.POINTER(Schedule)],
     "ctr_set_kernel_timing": [_P, _I],
     "ctr_kernel_timing": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)],
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
@@ -552,13 +563,22 @@ class Context:
         self._run(self.lib.ctr_osem_batched, (self.h, arr, n, it, int(use_graph),), "ctr_osem_batched")
 
     def autotune(self, n):
-        """Returns (rays per projection wave, waves per projection block, rows per backprojection thread)."""
-        rays, w, r = C.c_int(), C.c_int(), C.c_int()
-        self._run(self.lib.ctr_autotune, (self.h, n, C.byref(rays), C.byref(w), C.byref(r),), "ctr_autotune")
-        return rays.value, w.value, r.value
+        """Times the schedules for side n and keeps the fastest.  Returns a dict:
+        fp_rays (0 = LDS-tiled projector), fp_waves, fp_tile (samples per LDS
+        tile), bp_rows, and the chosen and default candidates' launch times."""
+        out = Schedule()
+        self._run(self.lib.ctr_autotune, (self.h, n, C.byref(out)), "ctr_autotune")
+        return out.as_dict()
 
-    def set_schedule(self, n, fp_rays, fp_waves, bp_rows):
-        self._run(self.lib.ctr_set_schedule, (self.h, n, fp_rays, fp_waves, bp_rows,), "ctr_set_schedule")
+    def set_schedule(self, n, fp_rays, fp_waves, bp_rows, fp_tile=60):
+        sch = Schedule(fp_rays, fp_waves, fp_tile, bp_rows, 0.0, 0.0, 0.0, 0.0)
+        self._run(self.lib.ctr_set_schedule, (self.h, n, C.byref(sch)), "ctr_set_schedule")
+
+    def get_schedule(self, n):
+        out = Schedule()
+        self._run(self.lib.ctr_get_schedule, (self.h, n, C.byref(out)), "ctr_get_schedule")
+        d = out.as_dict()
+        return {k: d[k] for k in ("fp_rays", "fp_waves", "fp_tile", "bp_rows")}
 
     def set_kernel_timing(self, on):
         self._run(self.lib.ctr_set_kernel_timing, (self.h, int(bool(on)),), "ctr_set_kernel_timing")

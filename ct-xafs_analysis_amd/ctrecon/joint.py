@@ -85,7 +85,10 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
     offsets: groups is a list of (context, y) with y [nz][pa][px] in that
     context's geometry (sides may differ between groups; pa must not).  The
     offset gradient and the Gauss-Newton curvature are summed over every
-    slice of every group.  Returns (xs, shifts, center, history)."""
+    slice of every group.  Returns (xs, shifts, center, history).
+    callback(k, xs, offsets, data), if given, runs after iteration k with the
+    data term as a 0-d device tensor, so that a callback that does not read
+    it makes the host wait for nothing."""
     pa = groups[0][0].geom.pa
     if any(lib.geom.pa != pa for lib, _ in groups):
         raise ValueError("all groups need the same projection count")
@@ -100,7 +103,7 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
     for k in range(iters):
         o = off.detach().requires_grad_(align and k >= warmup)
         h = torch.zeros(pa, dtype=torch.float32, device=dev)
-        data = 0.0
+        data = torch.zeros((), dtype=torch.float64, device=dev)
         new_x = []
         for G in st:
             zr = G.z.detach().requires_grad_(True)
@@ -109,7 +112,7 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
             loss = 0.5 * (r * r).sum() if G.rw is None else 0.5 * (G.rw * r * r).sum()
             loss.backward()        # the offset gradient accumulates over the groups
             with torch.no_grad():
-                data += float(0.5 * (r * r).sum())
+                data += 0.5 * (r * r).sum(dtype=torch.float64)
                 v = (G.z - (G.step * zr.grad if G.cw is None else G.cw * zr.grad)).contiguous()
                 x_new = torch.empty_like(v)
                 G.lib.tv_prox_2d(v, x_new, G.lam, 0.0)
@@ -119,7 +122,7 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
                     dp = torch.zeros_like(pd)
                     dp[..., 1:-1] = 0.5 * (pd[..., 2:] - pd[..., :-2])
                     h += (dp * dp).sum(dim=(0, 2)) if G.rw is None else (G.rw * dp * dp).sum(dim=(0, 2))
-        hist.append(data)
+        hist.append(data)          # stays on the device: no host wait inside an iteration
         with torch.no_grad():
             if o.requires_grad:
                 d = -o.grad / (h + mu * h.mean())
@@ -132,7 +135,7 @@ def joint_fista_tv_groups(groups, *, iters=100, tv=1.0e-3, align=True, warmup=10
         if callback is not None:
             callback(k, [G.x for G in st], off, hist[-1])
     center = off.mean()
-    return [G.x for G in st], off - center, center, hist
+    return [G.x for G in st], off - center, center, torch.stack(hist).tolist() if hist else []
 
 
 def joint_fista_tv(lib, y, *, iters=100, tv=1.0e-3, align=True, warmup=10, max_step=0.5, mu=1.0e-3,

@@ -334,19 +334,28 @@ ctr_status ctr_tv_prox_2d(ctr_ctx *ctx, const float *v, float *x_new, int dN, fl
 ctr_status ctr_osem_batched(ctr_ctx *ctx, const ctr_slice *slices, int count, int it,
                             int use_graph);
 
-/* Times the kernel variants for slice side n on the context's device and
- * keeps the fastest for that side: the forward-projection schedule (fp_rays
- * 0 = LDS-tiled blocks of 64 rays; 64, 32, 16 or 8 = rays per wave of the
- * direct-gather kernel, the other 64/rays lanes of a ray taking interleaved
- * samples), waves per forward-projection block and pixel rows per
- * backprojection thread.
- * Returns the choice through the pointers (each may be NULL). */
-ctr_status ctr_autotune(ctr_ctx *ctx, int n, int *fp_rays, int *fp_waves, int *bp_rows);
+/* A projector / backprojector schedule for one slice side.  No reference
+ * counterpart: the reference fixes its work-group shape per compiled program
+ * (GPU.cpp:169-172, 538-547). */
+typedef struct ctr_schedule {
+    int fp_rays;    /* 0: the LDS-tiled projector; 64 / 32 / 16 / 8: the direct-gather one, rays per wave */
+    int fp_waves;   /* waves per projection block: 4 for the LDS-tiled projector, 2 or 4 otherwise */
+    int fp_tile;    /* samples per LDS tile: 32, 44 or 60 (64 tiles must cover the side) */
+    int bp_rows;    /* backprojection rows per thread: 1, 2, 4, 8 or 16 */
+    float fp_ms, bp_ms;                  /* ctr_autotune: the chosen candidates' launch times */
+    float fp_ms_default, bp_ms_default;  /* ... and the default schedule's (tile 60, 8 rows) */
+} ctr_schedule;
 
-/* Sets the schedule for slice side n explicitly (the values ctr_autotune
- * would choose): fp_rays in {0, 64, 32, 16, 8}, fp_waves in {2, 4},
- * bp_rows in {1, 2, 4, 8, 16}. */
-ctr_status ctr_set_schedule(ctr_ctx *ctx, int n, int fp_rays, int fp_waves, int bp_rows);
+/* Times the projector schedules (LDS tiles of 60 / 44 / 32 samples, direct
+ * gathers of 64 / 32 / 16 / 8 rays per wave with 2 or 4 waves per block) and
+ * the backprojection row counts on a batch-sized launch of side n, keeps the
+ * fastest for side n and writes it (with its and the default's times) to
+ * *best.  Blocks the host.  Batched stacks run each side's schedule. */
+ctr_status ctr_autotune(ctr_ctx *ctx, int n, ctr_schedule *best);
+/* Sets the schedule of side n (the time fields are ignored). */
+ctr_status ctr_set_schedule(ctr_ctx *ctx, int n, const ctr_schedule *schedule);
+/* The schedule side n runs (the defaults unless tuned or set). */
+ctr_status ctr_get_schedule(ctr_ctx *ctx, int n, ctr_schedule *schedule);
 
 /* Average duration (ms) of the last timed forward-projection and
  * backprojection launches recorded with ctr_set_kernel_timing(ctx, 1). */

@@ -11,7 +11,8 @@ from _util import dev, host, phantom_stack, relerr, say
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [(0, 4), (64, 2), (64, 4), (32, 2), (32, 4), (16, 2), (16, 4), (8, 2), (8, 4)]
+SCHEDULES = [(0, 4, 60), (0, 4, 44), (0, 4, 32), (64, 2, 60), (64, 4, 60), (32, 2, 60), (32, 4, 60), (16, 2, 60),
+             (16, 4, 60), (8, 2, 60), (8, 4, 60)]
 
 
 def _geom(ctx, nx, px, pa, ss, ny=None):
@@ -20,15 +21,15 @@ def _geom(ctx, nx, px, pa, ss, ny=None):
     return ang, O.geom(nx, px, pa, ss=ss, ny=ny, zp=ctx.geom.zp)
 
 
-@pytest.mark.parametrize("rays,waves", SCHEDULES)
-def test_every_projector_schedule(ctx, rays, waves):
+@pytest.mark.parametrize("rays,waves,tile", SCHEDULES)
+def test_every_projector_schedule(ctx, rays, waves, tile):
     n, pa, ss, nz = 96, 60, 6, 2
     ang, g = _geom(ctx, n, n, pa, ss)
     rng = np.random.default_rng(5)
     img = rng.uniform(0.0, 1.0, (nz, n, n)).astype(np.float32)
     for bp_rows in (1, 2, 4, 8, 16):
-        ctx.set_schedule(n, rays, waves, bp_rows)
-        say("schedule", rays, waves, bp_rows)
+        ctx.set_schedule(n, rays, waves, bp_rows, fp_tile=tile)
+        say("schedule", rays, waves, tile, bp_rows)
         out = dev(np.zeros((nz, pa, n)))
         ctx.op("aart1", dev(img), dev(np.zeros((nz, pa, n))), out, nz, 1)
         assert relerr(host(out), O.aart1(g, ang, img, np.zeros((nz, pa, n), np.float32), 1)) < 1e-5
@@ -36,7 +37,8 @@ def test_every_projector_schedule(ctx, rays, waves):
         x = dev(np.ones((nz, n, n)))
         ctx.osem_execution(x, dev(sino), 2)
         e = relerr(host(x), O.osem_execution(g, ang, np.ones((nz, n, n), np.float32), sino, 2))
-        print(f"[parity] schedule rays={rays} waves={waves} rows={bp_rows}: OS-EM relerr {e:.3e}", flush=True)
+        print(f"[parity] schedule rays={rays} waves={waves} tile={tile} rows={bp_rows}: OS-EM relerr {e:.3e}",
+              flush=True)
         assert e < 1e-4
     ctx.set_schedule(n, 0, 4, 8)
 
@@ -152,6 +154,7 @@ def test_error_statuses_leave_the_context_usable(ctx):
                  ctx.fbp_execution(x, y.clone())),
         lambda: ctx.set_geometry(0, n, pa, ss, angles_deg=ang),           # empty grid
         lambda: ctx.set_schedule(n, 12, 4, 8),                             # no such schedule
+        lambda: ctx.set_schedule(3000, 0, 4, 8, fp_tile=44),               # 64 tiles of 44 samples < 3000
         lambda: ctx.osem_batched([], [], 1),                               # empty batch
         lambda: ctx.set_geometry(n, n, pa, ss, angles_deg=ang, ox=n + 2),  # output region beyond the grid
         lambda: (ctx.set_geometry(n, n, pa, ss, angles_deg=ang, ox=n - 8),  # output region: FBP only

@@ -7,6 +7,18 @@ something independent of its own formula:
   subset order            literal lists (GPU.cpp:564-582 worked by hand)
   forward projector       theta=0 closed form (column sums), translation
                           equivariance, 90-degree rotation symmetry, FP32 vs FP64
+  powerIter1              theta=0 / 90 closed forms of its half-texel
+                          geometry (no +0.5): (C[X-1] + C[X]) / 2 at theta=0
+  nearest backprojectors  single-angle probes on odd sizes: rounding vs
+  (OSEM2, AART2,          truncation, integer vs float halves, and each
+  backProjection,         update's closed form (x q, x + q a S/(A D),
+  FISTAbackProjection,    q S/(A D), x + q a S/(A D)/L 0.5, q S/A/D); update
+  powerIter2)             scales over a two-angle subset with S != A
+  FISTA3D momentum        literal beta', gamma and w for beta = 0, 1, 3, 10
+  backProjectionFBP       an impulse lands on exactly the line the +0.5
+                          offsets predict (theta = 0 and 90 degrees)
+  per-slice TV prox       brute-force L1-prox minimisation with four
+                          neighbours; slices independent
   alignment adjoint       dot-product test against the independently written
                           scatter adjoint (FP64)
   shift gradient          FP64 central finite differences of the forward model
