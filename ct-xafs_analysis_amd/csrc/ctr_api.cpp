@@ -1967,8 +1967,10 @@ ctr_status align_pad(ctr_ctx *c, const float *img, int dN)
 
 }  // namespace
 
-extern "C" ctr_status ctr_align_forward(ctr_ctx *c, const float *img, const float *shifts, float center, float *prj,
-                                        int dN)
+namespace {
+
+ctr_status align_forward(ctr_ctx *c, const float *img, const float *shifts, float center, float *prj, float *dprj,
+                         int dN)
 {
     TRY(op_prologue(c, img, prj, dN));
     if (!std::isfinite(center)) return fail(c, CTR_INVALID_ARGUMENT, "center is not finite");
@@ -1976,9 +1978,24 @@ extern "C" ctr_status ctr_align_forward(ctr_ctx *c, const float *img, const floa
     const ctr_geometry &g = c->g;
     const Layout l = make_layout(g.nx, g.ny, g.px);
     launch_align_forward(c->w_pimg.f(), c->w_pimgT.f(), l.ipitch, l.tpitch, static_cast<const float2 *>(c->cs.p),
-                         shifts, center, prj, dN, g.nx, g.ny, g.pa, g.px, c->stream);
+                         shifts, center, prj, dprj, dN, g.nx, g.ny, g.pa, g.px, c->stream);
     POST(c, "align forward");
     return CTR_OK;
+}
+
+}  // namespace
+
+extern "C" ctr_status ctr_align_forward(ctr_ctx *c, const float *img, const float *shifts, float center, float *prj,
+                                        int dN)
+{
+    return align_forward(c, img, shifts, center, prj, nullptr, dN);
+}
+
+extern "C" ctr_status ctr_align_forward_deriv(ctr_ctx *c, const float *img, const float *shifts, float center,
+                                              float *prj, float *dprj, int dN)
+{
+    if (c && !dprj) return fail(c, CTR_INVALID_ARGUMENT, "dprj is null");
+    return align_forward(c, img, shifts, center, prj, dprj, dN);
 }
 
 extern "C" ctr_status ctr_align_adjoint(ctr_ctx *c, const float *prj, const float *shifts, float center, float *img,

@@ -120,8 +120,10 @@ void launch_rows_to_full(const float *rows, float *full, int nz, int na, int rpi
                          int sub, int ss, hipStream_t stream);
 void launch_full_to_rows(const float *full, float *rows, int nz, int na, int rpitch, int roff, int pa, int px,
                          int sub, int ss, hipStream_t stream);
+// dprj, when not null, also receives each ray's derivative with respect to
+// its projection's offset.
 void launch_align_forward(const float *pimg, const float *pimgT, int ipitch, int tpitch, const float2 *cs,
-                          const float *shifts, float center, float *prj, int nz, int nx, int ny, int pa,
+                          const float *shifts, float center, float *prj, float *dprj, int nz, int nx, int ny, int pa,
                           int px, hipStream_t stream);
 // Zero rays on either side of each projection row in the adjoint's copy of
 // the rows, and that copy's size in floats (slice-fastest, in chunks of 1, 4

@@ -104,6 +104,7 @@ _SIGS = {
     "ctr_set_kernel_timing": [_P, _I],
     "ctr_kernel_timing": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)],
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
+    "ctr_align_forward_deriv": [_P, _P, _P, _F, _P, _P, _I],
     "ctr_align_adjoint": [_P, _P, _P, _F, _P, _I],
     "ctr_align_grad_shift": [_P, _P, _P, _P, _F, _P, _I],
     "ctr_merge_frames": [_P, _P, _I, _I, _I, C.c_longlong, _P],
@@ -602,6 +603,13 @@ class Context:
         dN = self._align_args(img, prj, shifts, "align_forward")
         self._run(self.lib.ctr_align_forward, (self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj), dN,),
                   "ctr_align_forward")
+
+    def align_forward_deriv(self, img, shifts, center, prj, dprj):
+        """align_forward, also writing dprj = d prj / d shifts[a] per ray."""
+        dN = self._align_args(img, prj, shifts, "align_forward_deriv")
+        self._sino(dprj, dN, "align_forward_deriv derivative rows")
+        self._run(self.lib.ctr_align_forward_deriv, (self.h, _ptr(img), _ptr(shifts), float(center), _ptr(prj),
+                                                   _ptr(dprj), dN,), "ctr_align_forward_deriv")
 
     def align_adjoint(self, prj, shifts, center, img):
         dN = self._align_args(img, prj, shifts, "align_adjoint")

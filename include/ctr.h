@@ -369,6 +369,14 @@ ctr_status ctr_kernel_timing(ctr_ctx *ctx, double *fp_ms, double *bp_ms, int *fp
  *   xi = (X - px/2) - shifts[a] - center.  shifts may be NULL (all zero). */
 ctr_status ctr_align_forward(ctr_ctx *ctx, const float *img, const float *shifts,
                              float center, float *prj, int dN);
+/* ctr_align_forward that also writes dprj[z][a][X], the derivative of
+ * prj[z][a][X] with respect to shifts[a] (and to center), from the same
+ * samples: sum over k of -cos(a) d/du + sin(a) d/dv of the bilinear sample.
+ * The shift gradient of a loss with incoming gradient g is then
+ * sum over z and X of g[z][a][X] * dprj[z][a][X] (what ctr_align_grad_shift
+ * gives by sampling the image again).  dprj has prj's shape. */
+ctr_status ctr_align_forward_deriv(ctr_ctx *ctx, const float *img, const float *shifts,
+                                   float center, float *prj, float *dprj, int dN);
 /* Adjoint of ctr_align_forward with respect to the image: the transpose of
  * the forward's bilinear sampling weights (1 - fract, fract) of each sample's
  * coordinates, exact except in the second image column / row, where one

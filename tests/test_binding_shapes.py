@@ -74,6 +74,7 @@ BAD_CALLS = {
     "align_forward shifts": lambda c: c.align_forward(z(1, 12, 16), z(11), 0.0, z(1, 12, 20)),
     "align_adjoint img": lambda c: c.align_adjoint(z(2, 12, 20), None, 0.0, z(1, 12, 16)),
     "align_grad gshift": lambda c: c.align_grad_shift(z(1, 12, 16), z(1, 12, 20), z(12), 0.0, z(13)),
+    "align_forward_deriv dprj": lambda c: c.align_forward_deriv(z(1, 12, 16), None, 0.0, z(1, 12, 20), z(1, 12, 19)),
     # sinogram formation
     "reslice shifts": lambda c: c.reslice(z(5, 4, 6), xshift=z(4)),
     "reslice out": lambda c: c.reslice(z(5, 4, 6), out=z(4, 5, 5)),

@@ -74,6 +74,60 @@ def test_dot_product_through_autograd(ctx):
         assert rel < 1e-6
 
 
+@pytest.mark.parametrize("nx,ny,px,pa", [(64, 64, 64, 90), (96, 40, 110, 36), (3900, 24, 72, 12)])
+def test_derivative_rows_match_recomputation(ctx, nx, ny, px, pa):
+    """ctr_align_forward_deriv (the forward writing each ray's offset
+    derivative) against ctr_align_forward and ctr_align_grad_shift (sampling
+    the image again): the projections are identical, sum_X g * dprj equals the
+    recomputed shift gradient, both are as close to the FP64 oracle as the
+    FP32 oracle is (or within 1e-4), and
+    autograd gives the same gradients either way.  The LDS-tiled kernel, and
+    the direct-gather one past 3840 columns."""
+    import torch
+    from ctrecon.autograd import project
+    rng = np.random.default_rng(nx + pa)
+    ang = (np.arange(pa) * (180.0 / pa) + 0.37).astype(np.float32)
+    ctx.set_geometry(nx, px, pa, 1, ny=ny, angles_deg=ang)
+    g = O.geom(nx, px, pa, ss=1, ny=ny)
+    img = rng.uniform(0, 1, (3, ny, nx)).astype(np.float32)
+    sh = rng.normal(0, 1.5, pa).astype(np.float32)
+    w = rng.normal(0, 1, (3, pa, px)).astype(np.float32)
+    center = -0.6
+    x, s_, wt = (torch.from_numpy(a).cuda() for a in (img, sh, w))
+    say("forward with derivative rows", nx, ny, px, pa)
+    p0 = torch.empty((3, pa, px), device="cuda")
+    p1, d1 = torch.empty_like(p0), torch.empty_like(p0)
+    ctx.align_forward(x, s_, center, p0)
+    ctx.align_forward_deriv(x, s_, center, p1, d1)
+    gs_re = torch.empty(pa, device="cuda")
+    ctx.align_grad_shift(x, wt, s_, center, gs_re)
+    torch.cuda.synchronize()
+    assert torch.equal(p0, p1)
+    gs_d = (wt * d1).sum(dim=(0, 2))
+    gs64 = O.align_grad(g, ang, img, w, sh, center, dtype=np.float64)
+    # FP32 coordinates near column 3900 carry ~2e-4 px of rounding, and a
+    # sample that changes texel cells changes its derivative by a step: the
+    # bound is twice the FP32 oracle's own distance from FP64 (or 1e-4)
+    bound = max(1e-4, 2.0 * relerr(O.align_grad(g, ang, img, w, sh, center, dtype=np.float32), gs64))
+    e_d, e_re = relerr(gs_d.cpu().numpy(), gs64), relerr(gs_re.cpu().numpy(), gs64)
+    e_dr = relerr(gs_d.cpu().numpy(), gs_re.cpu().numpy().astype(np.float64))
+    print(f"[grad] derivative rows vs FP64 {e_d:.2e}, recomputed vs FP64 {e_re:.2e} (bound {bound:.1e}), rows vs "
+          f"recomputed {e_dr:.2e}", flush=True)
+    assert e_d < bound and e_re < bound and e_dr < 1e-5
+    grads = []
+    for derivatives in (True, False):
+        xv = x.clone().requires_grad_(True)
+        sv = s_.clone().requires_grad_(True)
+        cv = torch.tensor(center, device="cuda", requires_grad=True)
+        (wt * project(ctx, xv, sv, cv, derivatives=derivatives)).sum().backward()
+        grads.append((xv.grad, sv.grad, cv.grad))
+    assert torch.equal(grads[0][0], grads[1][0])
+    assert relerr(grads[0][1].cpu().numpy(), grads[1][1].cpu().numpy().astype(np.float64)) < 1e-5
+    # the centre gradient sums the shift gradients, which partly cancel: bound
+    # its difference by the sum of their magnitudes
+    assert abs(float(grads[0][2] - grads[1][2])) <= 1e-5 * float(grads[1][1].abs().sum()) + 1e-6
+
+
 def test_chunked_stack_matches(ctx):
     import torch
     from ctrecon.autograd import project

@@ -39,8 +39,8 @@ def _smooth_image(n, seed):
 
 
 def test_align_kernels_2048sq_1800proj_vs_fp64(ctx):
-    """Forward, adjoint and shift gradient on one 2048^2 slice with all 1800
-    projections on the device (the LDS-tiled forward / gradient and the
+    """Forward, adjoint, shift gradient and the forward with derivative rows on
+    one 2048^2 slice with all 1800 projections on the device (the LDS-tiled forward / gradient and the
     pixel-driven adjoint at the workload's largest side), compared with the
     FP64 oracle on 12 of the angles: their forward rows and shift gradients
     depend on those angles alone, and an incoming gradient that is zero on
@@ -67,6 +67,12 @@ def test_align_kernels_2048sq_1800proj_vs_fp64(ctx):
     ctx.align_grad_shift(dev(img), dev(gp), dev(shifts), float(center), gsh)
     grad = host(gsh)
     assert np.all(grad[np.setdiff1d(np.arange(pa), pick)] == 0.0)
+    # the forward with derivative rows: the same projections, and the shift
+    # gradient as a reduction of the rows
+    out2, drows = dev(np.zeros((1, pa, n))), dev(np.zeros((1, pa, n)))
+    ctx.align_forward_deriv(dev(img), dev(shifts), float(center), out2, drows)
+    assert np.array_equal(host(out2)[0, pick], fwd)
+    grad_d = (gp[0, pick].astype(np.float64) * host(drows)[0, pick]).sum(axis=1)
     a_s, s_s, gp_s = ang[pick], shifts[pick], gp[:, pick]
     f64 = O.align_forward(gs_, a_s, img, s_s, float(center))
     a64 = O.align_adjoint(gs_, a_s, gp_s, s_s, float(center))
@@ -77,9 +83,11 @@ def test_align_kernels_2048sq_1800proj_vs_fp64(ctx):
     b_a = max(1e-5, 2.0 * relerr(O.align_adjoint(gs_, a_s, gp_s, s_s, float(center), dtype=np.float32), a64))
     b_g = max(1e-4, 2.0 * relerr(O.align_grad(gs_, a_s, img, gp_s, s_s, float(center), dtype=np.float32), g64))
     e_f, e_a, e_g = relerr(fwd, f64[0]), relerr(adj, a64), relerr(grad[pick], g64)
+    e_d = relerr(grad_d, g64)
     print(f"[parity] align 2048^2 x 1800 (12 angles vs FP64): forward {e_f:.2e} (bound {b_f:.1e}), adjoint "
-          f"{e_a:.2e} ({b_a:.1e}), shift gradient {e_g:.2e} ({b_g:.1e})", flush=True)
-    assert e_f < b_f and e_a < b_a and e_g < b_g
+          f"{e_a:.2e} ({b_a:.1e}), shift gradient {e_g:.2e} ({b_g:.1e}), from derivative rows {e_d:.2e}",
+          flush=True)
+    assert e_f < b_f and e_a < b_a and e_g < b_g and e_d < b_g
 
 
 def test_joint_alignment_on_c5_sized_groups():
