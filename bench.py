@@ -53,6 +53,12 @@ sys.path.insert(0, os.path.join(ROOT, "ct-xafs_analysis_amd"))
 METRIC = "batched OS-EM slice-iters/s vs per-slice launch, 1024² 1200 proj, relerr≤1e-4"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# LDS read rate of the device: conflict-free ds_read2_b32, 275 bytes per ns
+# per CU measured on MI355X (128 bytes per clock), over its 256 CUs.  The
+# projector reads 16 bytes of LDS per bilinear sample (two ds_read2_b32 of
+# two texels each).
+LDS_READ_TBS = 275.0 * 256 / 1e3
+FP_LDS_BYTES_PER_SAMPLE = 16.0
 # Bytes moved past L2 per forward-projection launch, per slice of 1024^2 with
 # 120 angles (one subset of 1200): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
 # the 128-slice k_fp_lds<0> launch, scaled by the same counters on a plain
@@ -464,6 +470,7 @@ def main():
     fp_flops = sum(14.0 * m * na * m for m in sides)          # 14 FLOP per bilinear sample, D = m samples per ray
     fp_bytes = sum(4 * m * m + 8 * na * m for m in sides)    # image once, measured rows in, ratio rows out
     fp_tflops = fp_flops / (fp_ms * 1e-3) / 1e12
+    fp_lds_tbs = sum(FP_LDS_BYTES_PER_SAMPLE * m * na * m for m in sides) / (fp_ms * 1e-3) / 1e12
     step_flops = sum(14.0 * m * pa * m + 2.0 * m * m * pa for m in sides)
     hbm_bytes = sum(12.0 * ss * m * m + 12.0 * m * pa for m in sides)
     uniform_1024 = args.workload == "uniform" and (args.n, pa, ss) == FP_TRAFFIC_SHAPE
@@ -477,6 +484,10 @@ def main():
         "launches_per_subset": nfp // ss,
         "whole_step_tflops": round(step_flops * args.steps / dt / 1e12, 3),
         "whole_step_hbm_frac": round(hbm_bytes * args.steps / dt / (PEAK_HBM_GBS * 1e9), 5),
+        # the bound the projector meets first: its LDS reads against the
+        # device's measured conflict-free read rate
+        "lds": {"achieved": round(fp_lds_tbs, 3), "peak": LDS_READ_TBS, "unit": "TB/s",
+                "frac": round(fp_lds_tbs / LDS_READ_TBS, 4)},
     }
     if args.workload == "uniform":
         workload = (f"batched OS-EM, {B} slices of {args.n}x{args.n}, {pa} projections over [0,180), {args.n} bins, "
