@@ -15,8 +15,23 @@ per-projection fractional x / y shifts; value = algorithmic GB/s
 --workload joint runs the batched joint image + alignment reconstruction
 (ctrecon.joint, FISTA-TV with per-projection detector offsets) on 96 uneven
 slices with sides from {512, ..., 2048}, 1800 projections, shifts N(0, 2) px
-and a centre offset U(-3, 3); a step is one joint iteration over all slices;
-value = iterations/s after the set-up (the power-iteration step sizes).
+and a centre offset U(-3, 3), each slice reconstructed on a grid 1.45x its
+detector width (the phantoms reach beyond the detector's square); a step is
+one joint iteration over all slices; value = iterations/s after the set-up
+(the power-iteration step sizes).
+
+--workload fbp times plain FBP of one 256 x 256 slice, 360 projections, 256
+bins, ZP = 512, a noiseless 10-ellipse phantom; a step is one FBP_execution
+call; value = slices/s.  --workload osem times OS-EM of one 512 x 512 slice,
+900 projections, 10 subsets, 20 iterations, a 30-ellipse phantom with
+Poisson noise at I0 = 1e4; a step is one 20-iteration OSEM_execution call;
+value = slice-iterations/s.  Both report the per-slice-launch form (a
+synchronisation after every kernel), parity against the oracle and the
+oracle's time on the host.  --workload fista runs the differentiable
+FISTA-TV with alignment (ctrecon.joint) on one 512 x 512 slice, 720
+projections, TV weight 1e-3, per-projection shifts N(0, 1.5) px and a
+centre offset U(-3, 3) in a noiseless sinogram, reconstructed on a 742 x
+742 grid; value = iterations/s over 100 iterations.
 
 --workload mixed runs the uneven CT-XAFS stack instead: E energies x S slices,
 each slice's side seeded from {256, 384, 512, 768, 1024} and shared by its
@@ -240,6 +255,156 @@ def reslice_main(args):
     ctx.close()
 
 
+def slice_main(args):
+    """One slice through one reference driver: FBP (256^2, 360 angles) or
+    OS-EM (512^2, 900 angles, 10 subsets, 20 iterations)."""
+    import statistics
+
+    import torch
+
+    from ctrecon import Context
+    from ctrecon import phantom as ph
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    fbp = args.workload == "fbp"
+    if fbp:
+        n, pa, ss, it, zp = 256, 360, 1, 1, 512
+        _, sino, ang = ph.make_slice(n, pa, 10, 1)
+    else:
+        n, pa, ss, it, zp = 512, 900, 10, 20, None
+        _, sino, ang = ph.make_slice(n, pa, 30, 2, i0=1e4)
+    y = torch.from_numpy(sino[None].copy()).cuda()
+    x0 = np.zeros((1, n, n), np.float32) if fbp else np.ones((1, n, n), np.float32)
+
+    def run(ctx, x):
+        if fbp:
+            ctx.fbp_execution(x, y.clone())
+        else:
+            ctx.osem_execution(x, y.clone(), it)
+
+    def timed(sync_each):
+        ctx = Context(0)
+        ctx.set_geometry(n, n, pa, ss, zp=zp, angles_deg=ang)
+        ctx.set_sync_each_kernel(sync_each)
+        xs = [torch.from_numpy(x0.copy()).cuda() for _ in range(args.warmup + args.steps)]
+        for k in range(args.warmup):
+            run(ctx, xs[k])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            run(ctx, xs[args.warmup + k])
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        out = xs[-1].cpu().numpy()
+        ctx.close()
+        return dt, out
+
+    dt, got = timed(False)
+    dt_each, _ = timed(True)
+    g = O.geom(n, n, pa, ss=ss, zp=zp)
+    if fbp:
+        c32, c64 = O.fbp_execution(g, ang, sino[None]), O.fbp_execution(g, ang, sino[None], dtype=np.float64)
+    else:
+        c32 = O.osem_execution(g, ang, x0.copy(), sino[None].copy(), it)
+        c64 = O.osem_execution(g, ang, x0.copy(), sino[None].copy(), it, dtype=np.float64)
+    rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))  # noqa: E731
+    times = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        if fbp:
+            O.fbp_execution(g, ang, sino[None])
+        else:
+            O.osem_execution(g, ang, x0.copy(), sino[None].copy(), it)
+        times.append(time.perf_counter() - t1)
+    t_cpu = statistics.median(times)
+    unit = "slices/s" if fbp else "slice-iters/s"
+    per = 1 if fbp else it
+    if fbp:
+        workload = f"FBP_execution of one {n}x{n} slice, {pa} projections over [0,180), {n} bins, ZP={zp}, Ram-Lak"
+        data = "synthetic: seeded noiseless 10-ellipse phantom, analytic sinogram"
+    else:
+        workload = (f"OSEM_execution of one {n}x{n} slice, {pa} projections over [0,180), {n} bins, {ss} subsets, "
+                    f"{it} iterations from a uniform 1.0 image")
+        data = "synthetic: seeded 30-ellipse phantom, analytic sinogram with Poisson noise at I0=1e4"
+    out = {
+        "metric": f"{'FBP' if fbp else 'OS-EM'} of one slice ({n}^2, {pa} projections)", "value": round(per / dt, 3),
+        "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": data,
+        "config": {"workload": workload + "; step = one driver call", "n": n, "pa": pa, "ss": ss, "iterations": it},
+        "per_slice_launch": {"value": round(per / dt_each, 3), "unit": unit, "ms_per_step": round(dt_each * 1e3, 3)},
+        "relerr_vs_fp32_compat": rel(got, c32), "relerr_vs_fp64": rel(got, c64),
+        "relerr_fp32_compat_vs_fp64": rel(c32, c64),
+        "roofline": None,
+        "cpu_baseline": {"value": round(per / t_cpu, 3), "unit": unit, "cores": O.num_threads(),
+                         "nproc": os.cpu_count(), "cpu_model": cpu_model(), "kind": "port",
+                         "sample": f"the same call in the oracle's FP32 compat restatement (C, OpenMP): 3 runs after "
+                                   f"the parity run, median {t_cpu * 1e3:.1f} ms"},
+    }
+    print(json.dumps(out))
+
+
+def fista_main(args):
+    """Differentiable FISTA-TV with alignment on one 512^2 slice, 720
+    projections (ctrecon.joint)."""
+    import torch
+
+    from ctrecon import Context
+    from ctrecon import phantom as ph
+    from ctrecon.joint import joint_fista_tv, joint_grid
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, pa, iters = 512, 720, 100
+    m = joint_grid(n)          # the phantom reaches beyond the detector's square
+    rng = np.random.default_rng(2027)
+    shifts = rng.normal(0.0, 1.5, pa)
+    center = rng.uniform(-3.0, 3.0)
+    _, sino, ang = ph.make_slice(n, pa, 10, 4, shifts=shifts, center=center)
+    ctx = Context(0)
+    ctx.set_geometry(m, n, pa, 1, ny=m, angles_deg=ang)
+    y = torch.from_numpy(sino[None].copy()).cuda()
+    th = np.deg2rad(ang.astype(np.float64))
+    B = np.stack([np.cos(th), np.sin(th)], 1)
+
+    def off_err(off):
+        e = off - (shifts + center)
+        coef, *_ = np.linalg.lstsq(B, e, rcond=None)
+        return float(np.sqrt(np.mean((e - B @ coef) ** 2)))
+
+    marks = [torch.cuda.Event(enable_timing=True)]
+    joint_fista_tv(ctx, y, iters=2, tv=1e-3)          # warm-up: kernels loaded, workspaces allocated
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    marks[0].record()
+
+    def cb(k, x, off, f):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        marks.append(ev)
+
+    x, s, c, hist = joint_fista_tv(ctx, y, iters=iters, tv=1e-3, callback=cb)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    iter_s = marks[1].elapsed_time(marks[-1]) / 1e3 / (len(marks) - 2)
+    e0, e1 = off_err(np.zeros(pa)), off_err((s + c).cpu().numpy().astype(np.float64))
+    log(f"FISTA-TV {n}^2 x {pa}: {iter_s * 1e3:.2f} ms per iteration, offset error {e0:.3f} -> {e1:.3f} px")
+    out = {
+        "metric": f"differentiable FISTA-TV iterations/s, one {n}^2 slice, {pa} projections",
+        "value": round(1.0 / iter_s, 3), "unit": "iterations/s", "n_gpus": 1, "steps": iters, "warmup": 2,
+        "ms_per_step": round(iter_s * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded noiseless 10-ellipse phantom, analytic sinogram with per-projection shifts "
+                "N(0, 1.5) px and a centre offset U(-3, 3)",
+        "config": {"workload": f"joint FISTA-TV (TV 1e-3) + detector-offset Gauss-Newton, one {n}x{n} slice, {pa} "
+                               f"projections ({n} bins), reconstructed on a {m}x{m} grid that holds the whole "
+                               f"phantom, {iters} iterations; step = one iteration",
+                   "grid": m, "total_s_with_setup": round(total, 3)},
+        "offset_error_px": {"start": round(e0, 4), "end": round(e1, 4), "modulo": "image translation"},
+        "data_term": {"first": hist[0], "last": hist[-1]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def joint_main(args):
     """Batched joint alignment over uneven slices (ctrecon.joint)."""
     import numpy as np
@@ -247,7 +412,7 @@ def joint_main(args):
 
     from ctrecon import Context
     from ctrecon import phantom as ph
-    from ctrecon.joint import joint_fista_tv_groups
+    from ctrecon.joint import joint_fista_tv_groups, joint_grid
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     pa, nsl = args.pa, 96
     rng = np.random.default_rng(2026)
@@ -264,7 +429,10 @@ def joint_main(args):
             _, sino, ang = ph.make_slice(n, pa, 10, 900 + n, shifts=shifts, center=center, i0=5e3, slice_index=k)
             ys.append(sino)
         c = Context(0)
-        c.set_geometry(n, n, pa, 1, angles_deg=ang)
+        # a grid that holds the whole phantom (it reaches beyond the square
+        # the detector width spans)
+        m = joint_grid(n)
+        c.set_geometry(m, n, pa, 1, ny=m, angles_deg=ang)
         groups.append((c, torch.from_numpy(np.stack(ys)).cuda()))
     torch.cuda.synchronize()
     log(f"joint: {nsl} slices {counts}, {pa} projections generated in {time.perf_counter() - t0:.1f} s")
@@ -306,8 +474,9 @@ def joint_main(args):
         "data": "synthetic: seeded 10-ellipse phantoms, analytic sinograms with per-projection shifts N(0, 2) px, "
                 "a centre offset U(-3, 3) and Poisson noise at I0=5e3",
         "config": {"workload": "joint FISTA-TV + detector-offset Gauss-Newton over uneven slices (one context per "
-                               "side); step = one iteration over all slices", "slices_per_side": counts, "pa": pa,
-                   "iterations": args.steps, "total_s_with_setup": round(total, 2)},
+                               "side, each slice reconstructed on a grid 1.45x its detector width, which holds the "
+                               "whole phantom); step = one iteration over all slices", "slices_per_side": counts,
+                   "pa": pa, "iterations": args.steps, "total_s_with_setup": round(total, 2)},
         "offset_error_px": {"start": round(e0, 4), "end": round(e1, 4), "modulo": "image translation"},
         "data_term": {"first": hist[0], "last": hist[-1]},
         "roofline": None, "cpu_baseline": None,
@@ -320,7 +489,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("uniform", "mixed", "reslice", "joint"), default="uniform")
+    ap.add_argument("--workload", choices=("uniform", "mixed", "reslice", "joint", "fbp", "osem", "fista"),
+                    default="uniform")
     ap.add_argument("--batch", type=int, default=128, help="uniform: slices in the batch")
     ap.add_argument("--n", type=int, default=None, help="uniform: slice side (1024); reslice: camera side (2048)")
     ap.add_argument("--energies", type=int, default=64, help="mixed: energies per slice")
@@ -344,6 +514,10 @@ def main():
     if args.workload == "joint":
         args.pa = 1800 if args.pa is None else args.pa
         return joint_main(args)
+    if args.workload in ("fbp", "osem"):
+        return slice_main(args)
+    if args.workload == "fista":
+        return fista_main(args)
     args.n = 1024 if args.n is None else args.n
     args.pa = 1200 if args.pa is None else args.pa
 

@@ -28,12 +28,24 @@ joint_fista_tv_groups runs several stacks of different slice sizes (one
 context each) against one set of offsets, as the slices of one CT-XAFS
 dataset share the per-projection detector motion: the offset gradient and
 curvature are summed over all slices of all groups.
+
+The image grid of a context may be larger than its detector (nx, ny > px),
+and should be whenever the object can reach beyond the square the detector
+width spans: every point of the object projects onto the detector at some
+angles, so mass the grid cannot hold makes the data inconsistent with the
+model, and the solver absorbs the mismatch in the offsets.  joint_grid(px)
+gives a grid side that holds an object of radius 1.45 px / 2.
 """
 import math
 
 import torch
 
 from .autograd import project
+
+
+def joint_grid(px: int, pad: float = 1.45) -> int:
+    """An even image side of about pad * px for a detector of px bins."""
+    return int(round(pad * px / 2.0)) * 2
 
 
 def _norm2_sq(lib, nz, ny, nx, off, device, iters):
