@@ -354,7 +354,7 @@ def fista_main(args):
     from ctrecon.joint import joint_fista_tv, joint_grid
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n, pa, iters = 512, 720, 100
-    m = joint_grid(n)          # the phantom reaches beyond the detector's square
+    m = joint_grid(n, args.grid_pad)          # the phantom reaches beyond the detector's square
     rng = np.random.default_rng(2027)
     shifts = rng.normal(0.0, 1.5, pa)
     center = rng.uniform(-3.0, 3.0)
@@ -395,8 +395,9 @@ def fista_main(args):
         "data": "synthetic: seeded noiseless 10-ellipse phantom, analytic sinogram with per-projection shifts "
                 "N(0, 1.5) px and a centre offset U(-3, 3)",
         "config": {"workload": f"joint FISTA-TV (TV 1e-3) + detector-offset Gauss-Newton, one {n}x{n} slice, {pa} "
-                               f"projections ({n} bins), reconstructed on a {m}x{m} grid that holds the whole "
-                               f"phantom, {iters} iterations; step = one iteration",
+                               f"projections ({n} bins), reconstructed on a {m}x{m} grid"
+                               + (" that holds the whole phantom" if m >= 1.4 * n else "")
+                               + f", {iters} iterations; step = one iteration",
                    "grid": m, "total_s_with_setup": round(total, 3)},
         "offset_error_px": {"start": round(e0, 4), "end": round(e1, 4), "modulo": "image translation"},
         "data_term": {"first": hist[0], "last": hist[-1]},
@@ -431,7 +432,7 @@ def joint_main(args):
         c = Context(0)
         # a grid that holds the whole phantom (it reaches beyond the square
         # the detector width spans)
-        m = joint_grid(n)
+        m = joint_grid(n, args.grid_pad)
         c.set_geometry(m, n, pa, 1, ny=m, angles_deg=ang)
         groups.append((c, torch.from_numpy(np.stack(ys)).cuda()))
     torch.cuda.synchronize()
@@ -474,9 +475,11 @@ def joint_main(args):
         "data": "synthetic: seeded 10-ellipse phantoms, analytic sinograms with per-projection shifts N(0, 2) px, "
                 "a centre offset U(-3, 3) and Poisson noise at I0=5e3",
         "config": {"workload": "joint FISTA-TV + detector-offset Gauss-Newton over uneven slices (one context per "
-                               "side, each slice reconstructed on a grid 1.45x its detector width, which holds the "
-                               "whole phantom); step = one iteration over all slices", "slices_per_side": counts,
-                   "pa": pa, "iterations": args.steps, "total_s_with_setup": round(total, 2)},
+                               f"side, each slice reconstructed on a grid {args.grid_pad}x its detector width"
+                               + (", which holds the whole phantom" if args.grid_pad >= 1.4 else "")
+                               + "); step = one iteration over all slices", "slices_per_side": counts,
+                   "grid_pad": args.grid_pad, "pa": pa, "iterations": args.steps,
+                   "total_s_with_setup": round(total, 2)},
         "offset_error_px": {"start": round(e0, 4), "end": round(e1, 4), "modulo": "image translation"},
         "data_term": {"first": hist[0], "last": hist[-1]},
         "roofline": None, "cpu_baseline": None,
@@ -500,6 +503,8 @@ def main():
     ap.add_argument("--per-slice", type=int, default=4, help="uniform: slices timed in the per-slice-launch baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle check")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--grid-pad", type=float, default=1.45,
+                    help="joint / fista: reconstruction grid side over the detector width (1.0: the detector's square)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference size for HBM byte counters)")
     args = ap.parse_args()
