@@ -36,7 +36,7 @@ class AlignedProjection(torch.autograd.Function):
         c = float(center.detach())
         prj = torch.empty((img.shape[0], pa, px), dtype=torch.float32, device=img.device)
         dprj = None
-        if derivatives and (shifts.requires_grad or center.requires_grad):
+        if derivatives and (fctx.needs_input_grad[1] or fctx.needs_input_grad[2]):
             dprj = torch.empty_like(prj)
             lib.align_forward_deriv(img, sh, c, prj, dprj)
         else:
