@@ -74,7 +74,8 @@ def test_dot_product_through_autograd(ctx):
         assert rel < 1e-6
 
 
-@pytest.mark.parametrize("nx,ny,px,pa", [(64, 64, 64, 90), (96, 40, 110, 36), (3900, 24, 72, 12)])
+@pytest.mark.parametrize("nx,ny,px,pa", [(64, 64, 64, 90), (96, 40, 110, 36), (150, 150, 100, 40),
+                                         (3900, 24, 72, 12)])
 def test_derivative_rows_match_recomputation(ctx, nx, ny, px, pa):
     """ctr_align_forward_deriv (the forward writing each ray's offset
     derivative) against ctr_align_forward and ctr_align_grad_shift (sampling
@@ -126,6 +127,37 @@ def test_derivative_rows_match_recomputation(ctx, nx, ny, px, pa):
     # the centre gradient sums the shift gradients, which partly cancel: bound
     # its difference by the sum of their magnitudes
     assert abs(float(grads[0][2] - grads[1][2])) <= 1e-5 * float(grads[1][1].abs().sum()) + 1e-6
+
+
+def test_padded_grid_adjoint_and_gradients_vs_fp64(ctx):
+    """The joint solver's padded grids (image 1.45x the detector width): the
+    adjoint against the FP64 oracle's and the dot test <A x, y> = <x, A^T y>,
+    on a 146^2 grid with 100 bins and 48 angles."""
+    import torch
+    n, px, pa = 146, 100, 48
+    rng = np.random.default_rng(77)
+    ang = (np.arange(pa) * (180.0 / pa) + 0.37).astype(np.float32)
+    ctx.set_geometry(n, px, pa, 1, ny=n, angles_deg=ang)
+    g = O.geom(n, px, pa, ss=1, ny=n)
+    x = rng.uniform(0, 1, (2, n, n)).astype(np.float32)
+    y = rng.normal(0, 1, (2, pa, px)).astype(np.float32)
+    sh = rng.normal(0, 1.5, pa).astype(np.float32)
+    center = 0.9
+    say("padded grid adjoint", n, px, pa)
+    ax = torch.empty((2, pa, px), device="cuda")
+    aty = torch.empty((2, n, n), device="cuda")
+    xs, ys, shs = (torch.from_numpy(a).cuda() for a in (x, y, sh))
+    ctx.align_forward(xs, shs, center, ax)
+    ctx.align_adjoint(ys, shs, center, aty)
+    torch.cuda.synchronize()
+    a64 = O.align_adjoint(g, ang, y, sh, center, dtype=np.float64)
+    f64 = O.align_forward(g, ang, x, sh, center, dtype=np.float64)
+    e_a, e_f = relerr(aty.cpu().numpy(), a64), relerr(ax.cpu().numpy(), f64)
+    lhs = float((ax.double() * ys.double()).sum())
+    rhs = float((xs.double() * aty.double()).sum())
+    dot = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"[grad] padded grid: forward {e_f:.2e}, adjoint {e_a:.2e} vs FP64, dot test {dot:.2e}", flush=True)
+    assert e_f < 1e-5 and e_a < 1e-5 and dot < 1e-6
 
 
 def test_chunked_stack_matches(ctx):
