@@ -193,6 +193,8 @@ void *resolve_scratch(int device)
 
 struct ctr_ctx {
     int device = 0;
+    int cus = 256;           // compute units of the device
+    bool adaptive_rows = true;   // small stacks run with fewer backprojection rows per thread
     hipStream_t own = nullptr, stream = nullptr;
     bool sync_each = false;
     std::string err;
@@ -547,6 +549,25 @@ struct StackBind {
 
 void fill_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b);
 
+// Rows per backprojection thread for the slices of S: the side's schedule
+// (rows), halved while the launch would give the device fewer than 4 blocks
+// per compute unit (unless the context runs every stack as scheduled).  One
+// slice of 512^2 at 8 rows is 128 blocks; at 1 row, 1024.  Every row count
+// gives each pixel the same sum in the same order, so only the launch shape
+// changes.
+int fill_rows(const ctr_ctx *c, const SliceSet &S, int rows)
+{
+    if (!c->adaptive_rows) return rows;
+    const long long want = 4LL * c->cus;
+    auto blocks = [&](int r) {
+        long long n = 0;
+        for (const SliceDev &d : S.h) n += bp_blocks(d.nx, d.ny, r);
+        return n;
+    };
+    while (rows > 1 && blocks(rows) < want) rows /= 2;
+    return rows;
+}
+
 ctr_status build_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
 {
     fill_stack(c, S, nz, b);
@@ -580,6 +601,7 @@ void fill_stack(ctr_ctx *c, SliceSet &S, int nz, const StackBind &b)
         d.aux = b.aux ? b.aux + (size_t)z * g.nx * g.ny : nullptr;
         d.layer = b.layer_base + z;
     }
+    S.bp_rows = fill_rows(c, S, t.bp_rows);
 }
 
 // Device workspace for an nz-layer stack of the current geometry.
@@ -739,6 +761,11 @@ extern "C" ctr_status ctr_create(int device, ctr_ctx **out)
     if (hipSetDevice(device) != hipSuccess) return CTR_NO_DEVICE;
     auto *c = new ctr_ctx();
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cus = cus;
+    else
+        (void)hipGetLastError();
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return CTR_HIP_ERROR;
@@ -1724,7 +1751,10 @@ extern "C" ctr_status ctr_osem_batched(ctr_ctx *c, const ctr_slice *slices, int 
             ot += pimgT_floats(l);
             orr += (size_t)na * l.rpitch;
         }
-        for (auto &S : P.sets) TRY(upload_set(c, *S));
+        for (auto &S : P.sets) {
+            S->bp_rows = fill_rows(c, *S, S->bp_rows);
+            TRY(upload_set(c, *S));
+        }
     }
     BatchPlan &P = *c->plan;
     for (auto &S : P.sets) {
@@ -1814,6 +1844,16 @@ extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, const ctr_schedule *s)
     t.bp_rows = s->bp_rows;
     c->tune[n] = t;
     c->plan.reset();
+    return CTR_OK;
+}
+
+extern "C" ctr_status ctr_set_adaptive_rows(ctr_ctx *c, int enable)
+{
+    if (!c) return CTR_INVALID_ARGUMENT;
+    if (c->adaptive_rows != (enable != 0)) {
+        c->adaptive_rows = enable != 0;
+        c->plan.reset();   // a cached batched plan holds its sets' row counts
+    }
     return CTR_OK;
 }
 

@@ -102,6 +102,7 @@ _SIGS = {
     "ctr_set_schedule": [_P, _I, C.POINTER(Schedule)],
     "ctr_get_schedule": [_P, _I, C.POINTER(Schedule)],
     "ctr_set_kernel_timing": [_P, _I],
+    "ctr_set_adaptive_rows": [_P, _I],
     "ctr_kernel_timing": [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)],
     "ctr_align_forward": [_P, _P, _P, _F, _P, _I],
     "ctr_align_forward_deriv": [_P, _P, _P, _F, _P, _P, _I],
@@ -580,6 +581,13 @@ class Context:
         self._run(self.lib.ctr_get_schedule, (self.h, n, C.byref(out)), "ctr_get_schedule")
         d = out.as_dict()
         return {k: d[k] for k in ("fp_rays", "fp_waves", "fp_tile", "bp_rows")}
+
+    def set_adaptive_rows(self, on):
+        """on (the default): stacks too small to give the device 4
+        backprojection blocks per compute unit run with fewer rows per thread
+        than their side's schedule; off: every stack runs its schedule as set.
+        The images are the same either way."""
+        self._run(self.lib.ctr_set_adaptive_rows, (self.h, int(bool(on)),), "ctr_set_adaptive_rows")
 
     def set_kernel_timing(self, on):
         self._run(self.lib.ctr_set_kernel_timing, (self.h, int(bool(on)),), "ctr_set_kernel_timing")

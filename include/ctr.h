@@ -341,7 +341,8 @@ typedef struct ctr_schedule {
     int fp_rays;    /* 0: the LDS-tiled projector; 64 / 32 / 16 / 8: the direct-gather one, rays per wave */
     int fp_waves;   /* waves per projection block: 4 for the LDS-tiled projector, 2 or 4 otherwise */
     int fp_tile;    /* samples per LDS tile: 32, 44 or 60 (64 tiles must cover the side) */
-    int bp_rows;    /* backprojection rows per thread: 1, 2, 4, 8 or 16 */
+    int bp_rows;    /* backprojection rows per thread: 1, 2, 4, 8 or 16 (small stacks may
+                       run with fewer: ctr_set_adaptive_rows) */
     float fp_ms, bp_ms;                  /* ctr_autotune: the chosen candidates' launch times */
     float fp_ms_default, bp_ms_default;  /* ... and the default schedule's (tile 60, 8 rows) */
 } ctr_schedule;
@@ -356,6 +357,13 @@ ctr_status ctr_autotune(ctr_ctx *ctx, int n, ctr_schedule *best);
 ctr_status ctr_set_schedule(ctr_ctx *ctx, int n, const ctr_schedule *schedule);
 /* The schedule side n runs (the defaults unless tuned or set). */
 ctr_status ctr_get_schedule(ctr_ctx *ctx, int n, ctr_schedule *schedule);
+/* enable != 0 (the default): a stack whose backprojection would have fewer
+ * than 4 blocks per compute unit at its side's bp_rows runs with bp_rows
+ * halved until it has them or reaches 1 (one 512^2 slice: 1 row, 1024
+ * blocks).  0: every stack runs its side's bp_rows as set.  The images are
+ * the same either way (every row count sums each pixel's angles in the same
+ * order); only the launch shape changes.  No reference counterpart. */
+ctr_status ctr_set_adaptive_rows(ctr_ctx *ctx, int enable);
 
 /* Average duration (ms) of the last timed forward-projection and
  * backprojection launches recorded with ctr_set_kernel_timing(ctx, 1). */

@@ -27,20 +27,55 @@ def test_every_projector_schedule(ctx, rays, waves, tile):
     ang, g = _geom(ctx, n, n, pa, ss)
     rng = np.random.default_rng(5)
     img = rng.uniform(0.0, 1.0, (nz, n, n)).astype(np.float32)
-    for bp_rows in (1, 2, 4, 8, 16):
-        ctx.set_schedule(n, rays, waves, bp_rows, fp_tile=tile)
-        say("schedule", rays, waves, tile, bp_rows)
-        out = dev(np.zeros((nz, pa, n)))
-        ctx.op("aart1", dev(img), dev(np.zeros((nz, pa, n))), out, nz, 1)
-        assert relerr(host(out), O.aart1(g, ang, img, np.zeros((nz, pa, n), np.float32), 1)) < 1e-5
-        _, sino, _ = phantom_stack(nz, n, pa, 6, seed=8, i0=1e4)
-        x = dev(np.ones((nz, n, n)))
-        ctx.osem_execution(x, dev(sino), 2)
-        e = relerr(host(x), O.osem_execution(g, ang, np.ones((nz, n, n), np.float32), sino, 2))
-        print(f"[parity] schedule rays={rays} waves={waves} tile={tile} rows={bp_rows}: OS-EM relerr {e:.3e}",
-              flush=True)
-        assert e < 1e-4
-    ctx.set_schedule(n, 0, 4, 8)
+    ctx.set_adaptive_rows(False)          # run each row count as set, small as the stack is
+    try:
+        for bp_rows in (1, 2, 4, 8, 16):
+            ctx.set_schedule(n, rays, waves, bp_rows, fp_tile=tile)
+            say("schedule", rays, waves, tile, bp_rows)
+            out = dev(np.zeros((nz, pa, n)))
+            ctx.op("aart1", dev(img), dev(np.zeros((nz, pa, n))), out, nz, 1)
+            assert relerr(host(out), O.aart1(g, ang, img, np.zeros((nz, pa, n), np.float32), 1)) < 1e-5
+            _, sino, _ = phantom_stack(nz, n, pa, 6, seed=8, i0=1e4)
+            x = dev(np.ones((nz, n, n)))
+            ctx.osem_execution(x, dev(sino), 2)
+            e = relerr(host(x), O.osem_execution(g, ang, np.ones((nz, n, n), np.float32), sino, 2))
+            print(f"[parity] schedule rays={rays} waves={waves} tile={tile} rows={bp_rows}: OS-EM relerr {e:.3e}",
+                  flush=True)
+            assert e < 1e-4
+    finally:
+        ctx.set_schedule(n, 0, 4, 8)
+        ctx.set_adaptive_rows(True)
+
+
+@pytest.mark.parametrize("n,nz", [(512, 1), (96, 3), (1024, 1)])
+def test_adaptive_rows_match_the_set_row_count(ctx, n, nz):
+    """A stack too small for 4 backprojection blocks per compute unit at the
+    side's 8 rows runs with fewer rows per thread (one 512^2 slice: 1 row).
+    The images equal those of the 8-row launch bit for bit, for OS-EM, AART
+    and the batched driver."""
+    pa, ss = 60, 6
+    ang, g = _geom(ctx, n, n, pa, ss)
+    _, sino, _ = phantom_stack(nz, n, pa, 6, seed=11, i0=1e4)
+    y = dev(sino)
+    res = {}
+    try:
+        for adaptive in (True, False):
+            ctx.set_adaptive_rows(adaptive)
+            say("adaptive rows", adaptive, n, nz)
+            x = dev(np.ones((nz, n, n)))
+            ctx.osem_execution(x, y.clone(), 2)
+            a = dev(np.zeros((nz, n, n)))
+            ctx.aart_execution(a, y.clone(), 1)
+            imgs = [dev(np.ones((n, n))) for _ in range(nz)]
+            ctx.osem_batched(imgs, [y[z] for z in range(nz)], 2, use_graph=False)
+            res[adaptive] = (host(x), host(a), np.stack([host(i) for i in imgs]))
+    finally:
+        ctx.set_adaptive_rows(True)
+    for got, want, what in zip(res[True], res[False], ("OS-EM", "AART", "batched OS-EM")):
+        assert np.array_equal(got, want), what
+    # and the 8-row launch is the oracle's OS-EM
+    e = relerr(res[False][0], O.osem_execution(g, ang, np.ones((nz, n, n), np.float32), sino, 2))
+    assert e < 1e-4
 
 
 @pytest.mark.parametrize("nx,ny,px,pa,ss", [
