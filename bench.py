@@ -55,6 +55,8 @@ Printed beside it:
                      nproc and the CPU model
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -64,6 +66,23 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ct-xafs_analysis_amd"))
+
+
+@contextlib.contextmanager
+def no_gc():
+    """A timed loop without Python's cyclic garbage collector: one collection
+    before it and none inside, as timeit does.  With torch loaded a full
+    collection stalls the host for tens of milliseconds, which a loop of
+    asynchronous calls would count as device time (one such pause in 50 FBP
+    calls of 0.08 ms made them read 0.87 ms)."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 METRIC = "batched OS-EM slice-iters/s vs per-slice launch, 1024² 1200 proj, relerr≤1e-4"
 PEAK_FP32_TFLOPS = 157.3
@@ -206,12 +225,13 @@ def reslice_main(args):
         ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
     ctx.synchronize()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.reslice(mt, xs, ys, baseup=0.1, out=prj)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     nbytes = 8.0 * A * H * W
     value = nbytes * args.steps / dt / 1e9
     achieved = nbytes / (k_ms * 1e-3) / 1e9
@@ -291,11 +311,12 @@ def slice_main(args):
         for k in range(args.warmup):
             run(ctx, xs[k])
         ctx.synchronize()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            run(ctx, xs[args.warmup + k])
-        ctx.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+        with no_gc():
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                run(ctx, xs[args.warmup + k])
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
         out = xs[-1].cpu().numpy()
         ctx.close()
         return dt, out
@@ -559,11 +580,12 @@ def main():
         # timed as the tuned run below
         ctx.osem_batched(imgs, sinos, 1, use_graph=True)
         ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ctx.osem_batched(imgs, sinos, 1, use_graph=True)
-        ctx.synchronize()
-        default_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+            ctx.synchronize()
+            default_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         iters_done += 1 + args.steps
         log(f"default schedule: {default_ms:.1f} ms per step")
         # then every side's own schedule (each side of a batched stack runs its own)
@@ -603,15 +625,16 @@ def main():
         ctx.osem_batched(imgs, sinos, 1, use_graph=True)
     ctx.synchronize()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        ctx.osem_batched(imgs, sinos, 1, use_graph=True)
-        if args.workload == "mixed":
-            ctx.synchronize()
-            log(f"step {s + 1}/{args.steps} done")
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            ctx.osem_batched(imgs, sinos, 1, use_graph=True)
+            if args.workload == "mixed":
+                ctx.synchronize()
+                log(f"step {s + 1}/{args.steps} done")
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     iters_done += args.warmup + args.steps
     gpu_slice0 = imgs[0].cpu().numpy()[None]        # after every iteration of the run
     value = eq * args.steps / dt
@@ -635,10 +658,13 @@ def main():
             y = sinos[i].view(1, pa, m)
             pctx.osem_execution(x.clone(), y.clone(), 1)          # warm
             pctx.synchronize()
-            t1 = time.perf_counter()
-            pctx.osem_execution(x, y.clone(), 1)
-            pctx.synchronize()
-            t_side.setdefault(m, []).append(time.perf_counter() - t1)
+            yc = y.clone()
+            torch.cuda.synchronize()
+            with no_gc():
+                t1 = time.perf_counter()
+                pctx.osem_execution(x, yc, 1)
+                pctx.synchronize()
+                t_side.setdefault(m, []).append(time.perf_counter() - t1)
     t_mean = {m: sum(v) / len(v) for m, v in t_side.items()}
     per_slice = eq / sum(t_mean[m] for m in sides)
     log(f"per-slice launch: {per_slice:.2f} (1024^2-equivalent) slice-iters/s")
