@@ -1852,7 +1852,13 @@ extern "C" ctr_status ctr_set_adaptive_rows(ctr_ctx *c, int enable)
     if (!c) return CTR_INVALID_ARGUMENT;
     if (c->adaptive_rows != (enable != 0)) {
         c->adaptive_rows = enable != 0;
-        c->plan.reset();   // a cached batched plan holds its sets' row counts
+        if (c->plan) {
+            // a cached batched plan holds its sets' row counts: rebuild it,
+            // once its graph and buffers are no longer in use
+            HIPTRY(c, hipSetDevice(c->device));
+            HIPTRY(c, hipStreamSynchronize(c->stream));
+            c->plan.reset();
+        }
     }
     return CTR_OK;
 }
