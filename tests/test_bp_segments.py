@@ -1,15 +1,15 @@
 """The backprojector stages, per angle, the bins between the extremes its
 block's four (clamped) corner pixels map to, and gathers from that LDS
-segment of at most 96 entries.  This restates the kernel's float32 bin
-arithmetic in numpy (no fused operations, the same order) and checks, for
-sampled blocks and angles of several geometries and row counts, that every
-thread's bin lies in [bin(min corner), bin(max corner)] and that the
-segment fits."""
+segment of at most 96 entries (68 with one row per thread).  This restates
+the kernel's float32 bin arithmetic in numpy (no fused operations, the same
+order) and checks, for sampled blocks and angles of several geometries and
+row counts, that every thread's bin lies in [bin(min corner), bin(max
+corner)] and that the segment fits."""
 import numpy as np
 import pytest
 
 f32 = np.float32
-SEG = 96
+SEG = {1: 68, 8: 96, 16: 96}      # entries per angle of the LDS segment, by rows per thread
 
 
 def _half(i, n, int_half):
@@ -26,8 +26,8 @@ def test_block_segments_hold_every_bin(nx, ny, px):
     S = np.sin(th).astype(f32)[:, None, None]
     C[np.abs(C) < 1e-7] = 0
     S[np.abs(S) < 1e-7] = 0
-    widest = 0
-    for R in (1, 8, 16):
+    widest = {R: 0 for R in SEG}
+    for R in SEG:
         tiles = [(tx, ty) for tx in range((nx + 63) // 64) for ty in range((ny + 4 * R - 1) // (4 * R))]
         if len(tiles) > 12:
             tiles = [tiles[0], tiles[-1]] + [tiles[i] for i in rng.choice(len(tiles), 10, replace=False)]
@@ -52,5 +52,6 @@ def test_block_segments_hold_every_bin(nx, ny, px):
                     b = binf(bx)
                     lo, hi = binf(cmin), binf(cmax)
                     assert np.all(b >= lo) and np.all(b <= hi), (R, tx, ty, int_half, add_half)
-                    widest = max(widest, int((hi - lo + 1).max()))
-    assert widest <= SEG
+                    widest[R] = max(widest[R], int((hi - lo + 1).max()))
+    for R, w in widest.items():
+        assert w <= SEG[R], (R, w)
