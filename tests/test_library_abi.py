@@ -50,6 +50,8 @@ def test_host_only_entry_points():
     # null context -> CTR_INVALID_ARGUMENT, never a crash
     assert lib.ctr_synchronize(None) == 1
     assert lib.ctr_set_geometry(None, None, None, None) == 1
+    for setter in ("ctr_set_sync_each_kernel", "ctr_set_kernel_timing", "ctr_set_adaptive_rows"):
+        assert getattr(lib, setter)(None, 1) == 1, setter
     lib.ctr_last_error.restype = C.c_char_p
     assert lib.ctr_last_error(None) == b"null context"
 
