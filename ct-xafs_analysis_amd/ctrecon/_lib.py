@@ -216,11 +216,20 @@ class Context:
     """One library context (device + stream).  Methods take float32 GPU tensors
     and run on the context stream, ordered after the work queued on torch's
     current stream and before the work queued there afterwards (as a torch
-    operation would be).  synchronize() waits for the context stream."""
+    operation would be).  synchronize() waits for the context stream.
+
+    use_torch_stream=True runs the library on torch's current stream instead
+    of a stream of its own.  That stream must not be torch's default stream:
+    its handle is null, which ctr_set_stream takes to mean the context's own
+    stream (ValueError)."""
 
     def __init__(self, device: int = 0, use_torch_stream: bool = False):
         import torch
         self._torch = torch
+        if use_torch_stream and torch.cuda.current_stream(device).cuda_stream == 0:
+            raise ValueError("use_torch_stream: torch's current stream is the default stream, whose null handle "
+                             "ctr_set_stream reads as the context's own stream; run under a torch.cuda.Stream "
+                             "(calls are ordered with torch's stream either way)")
         self.lib = load()
         h = C.c_void_p()
         st = self.lib.ctr_create(device, C.byref(h))

@@ -54,3 +54,47 @@ def test_two_drivers_queued_back_to_back():
     assert np.isfinite(host(xa)).all()
     ctx.close()
     ref.close()
+
+
+def test_drivers_on_a_caller_stream():
+    """ctr_set_stream: a context created on torch's current stream (a
+    torch.cuda.Stream; the default stream's null handle is refused) and one
+    given a separate torch stream give the images of a context on its own
+    stream, bit for bit; resetting the stream restores the library's own."""
+    import torch
+    from ctrecon import Context
+    from _util import phantom_stack
+    n, pa, ss, nz = 128, 60, 6, 2
+    ang = (np.arange(pa) * (180.0 / pa)).astype(np.float32)
+    _, sino, _ = phantom_stack(nz, n, pa, 6, seed=31, i0=1e4)
+    y = torch.from_numpy(sino).cuda()
+
+    def run(cx):
+        cx.set_geometry(n, n, pa, ss, angles_deg=ang)
+        x = torch.ones((nz, n, n), device="cuda")
+        cx.osem_execution(x, y.clone(), 2)
+        cx.synchronize()
+        torch.cuda.synchronize()
+        return x
+
+    own = Context(0)
+    want = run(own)
+    say("contexts on a caller's stream")
+    with pytest.raises(ValueError):
+        Context(0, use_torch_stream=True)     # under torch's default stream
+    ts = torch.cuda.Stream()
+    with torch.cuda.stream(ts):
+        on_torch = Context(0, use_torch_stream=True)
+        assert on_torch.library_stream().cuda_stream == ts.cuda_stream
+        got_torch = run(on_torch)
+    side = torch.cuda.Stream()
+    on_side = Context(0)
+    on_side.set_stream(side.cuda_stream)
+    assert on_side.library_stream().cuda_stream == side.cuda_stream
+    got_side = run(on_side)
+    on_side.set_stream(None)                  # back to the context's own stream
+    assert on_side.library_stream().cuda_stream not in (side.cuda_stream, torch.cuda.current_stream().cuda_stream)
+    got_own_again = run(on_side)
+    assert torch.equal(got_torch, want) and torch.equal(got_side, want) and torch.equal(got_own_again, want)
+    for cx in (own, on_torch, on_side):
+        cx.close()
