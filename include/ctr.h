@@ -84,7 +84,8 @@ const char *ctr_last_error(const ctr_ctx *ctx);
 const char *ctr_status_string(ctr_status status);
 
 /* Uses `hip_stream` (a hipStream_t) instead of the context's own stream;
- * NULL restores the context stream. */
+ * NULL restores the context stream (so the null default stream cannot be
+ * selected: pass a created stream). */
 ctr_status ctr_set_stream(ctr_ctx *ctx, void *hip_stream);
 /* Writes the hipStream_t the context's work runs on (its own stream, or the
  * one given to ctr_set_stream), so a caller can order its own streams with it.
