@@ -1843,7 +1843,13 @@ extern "C" ctr_status ctr_set_schedule(ctr_ctx *c, int n, const ctr_schedule *s)
     t.fp_tile = s->fp_rays == 0 ? s->fp_tile : 60;
     t.bp_rows = s->bp_rows;
     c->tune[n] = t;
-    c->plan.reset();
+    if (c->plan) {
+        // the cached batched plan holds the old schedule: rebuild it, once
+        // its graph and buffers are no longer in use
+        HIPTRY(c, hipSetDevice(c->device));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+        c->plan.reset();
+    }
     return CTR_OK;
 }
 

@@ -20,9 +20,9 @@
  *    work), ctr_autotune (it times kernels), ctr_osem_batched when its slice
  *    list differs from the previous call's (it rebuilds its plan), any call
  *    that must grow a workspace buffer (the old one is freed, which waits
- *    for the device), ctr_set_adaptive_rows when it changes the setting
- *    after a batched call, ctr_set_sync_each_kernel(1) mode, and kernel
- *    timing.
+ *    for the device), ctr_set_schedule and ctr_set_adaptive_rows (when it
+ *    changes the setting) after a batched call, ctr_set_sync_each_kernel(1)
+ *    mode, and kernel timing.
  *    A context is thread-compatible, not thread-safe: use one context per
  *    host thread.
  *  - Errors are returned as ctr_status; ctr_last_error() gives the message.
