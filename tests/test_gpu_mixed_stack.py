@@ -93,3 +93,35 @@ def test_mixed_stack_per_side_schedules(ctx):
         e = relerr(imgs[i].cpu().numpy()[None], want)
         print(f"[parity] per-side schedule {m} {sched[m]}: relerr vs FP32 compat {e:.3e}", flush=True)
         assert e < 1e-5
+
+
+def test_batched_stack_past_the_lds_projector_side(ctx):
+    """A side above 3840 (one LDS tile of 60 samples per lane no longer
+    covers a ray) runs the direct-gather projector by default, in its own
+    slice set beside the LDS-tiled sides: every slice still equals its own
+    OSEM_execution bit for bit."""
+    import torch
+    from ctrecon import phantom as ph
+    pa, ss, it = 12, 3, 1
+    sides_s = [64, 3900, 200]
+    say("batched stack with a 3900^2 slice")
+    _, ang, sinos = ph.mixed_energy_stack_torch(1, len(sides_s), pa, seed=97, slice_sides=sides_s)
+    ctx.set_geometry(sides_s[0], sides_s[0], pa, ss, angles_deg=ang)
+    for m in sides_s:
+        if m <= 3840:
+            ctx.set_schedule(m, 0, 4, 8)
+        else:
+            with pytest.raises(Exception):
+                ctx.set_schedule(m, 0, 4, 8)  # no LDS tile covers the ray: refused
+    assert ctx.get_schedule(3900)["fp_rays"] > 0      # the default there gathers directly
+    imgs = [torch.ones((m, m), device="cuda") for m in sides_s]
+    ctx.osem_batched(imgs, sinos, it, use_graph=True)
+    ctx.synchronize()
+    for i, m in enumerate(sides_s):
+        ctx.set_geometry(m, m, pa, ss, angles_deg=ang)
+        x = torch.ones((1, m, m), device="cuda")
+        say("per-slice", m)
+        ctx.osem_execution(x, sinos[i].view(1, pa, m).clone(), it)
+        ctx.synchronize()
+        assert torch.equal(x[0], imgs[i]), m
+        assert torch.isfinite(imgs[i]).all(), m
